@@ -1,0 +1,348 @@
+"""ghcm_amd -- MI355X-native gHCM eikonal solver (Python host bindings).
+
+The solver itself lives in ``libghcm.so`` (gfx950 HIP kernels + C-ABI + C++ API,
+built from ``ghcm_amd/csrc``).  This module is a thin ctypes layer over the C-ABI of
+``include/ghcm.h`` and ``include/eikonal_c.h``; it mirrors the reference's solver
+surface (src/eikonal.h:12-67):
+
+    HcmConfig fields          -> Config(...) / Context(**config)
+    eikonal_hcm_opencl_setup  -> Context(ordinal, **config)
+    eikonal_hcm_opencl_ex     -> Context.solve(cost, seeds)
+    eikonal_fmm_sequential    -> fmm_sequential(cost, seeds)
+    eikonal_fim_sequential    -> fim_sequential(minimum_change, cost, seeds)
+
+There is no CPU fallback for the GPU path: if ``libghcm.so`` is missing, or no HIP
+device is present, the GPU entry points raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libghcm.so")
+
+# ghcm_status
+OK, ERR_INVALID_ARG, ERR_NO_DEVICE, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_CONVERGED, \
+    ERR_UNSUPPORTED, ERR_OVER_BUDGET, ERR_NOT_FOUND, ERR_STOPPED = range(10)
+# ghcm_kernel / ghcm_scheduler / ghcm_field_kind
+KERNEL_SYSTOLIC, KERNEL_FIM = 0, 1
+SCHED_DEVICE_TOPN, SCHED_HOST_PQ = 0, 1
+FIELD_COST_F32, FIELD_SPEED_F32, FIELD_SPEED_F16, FIELD_SPEED_BF16 = 0, 1, 2, 3
+
+KINDS = {"cost": FIELD_COST_F32, "speed_f32": FIELD_SPEED_F32,
+         "speed_f16": FIELD_SPEED_F16, "speed_bf16": FIELD_SPEED_BF16}
+
+
+class Config(ctypes.Structure):
+    """ghcm_config (include/ghcm.h); mirrors HcmConfig, src/eikonal.h:12-27."""
+    _fields_ = [("tile_w", ctypes.c_uint32), ("tile_h", ctypes.c_uint32),
+                ("workers_overload", ctypes.c_float),
+                ("compute_unit_overload", ctypes.c_float),
+                ("n_active", ctypes.c_uint32), ("min_change", ctypes.c_float),
+                ("cache_mode", ctypes.c_int32), ("kernel", ctypes.c_int32),
+                ("scheduler", ctypes.c_int32), ("inner_iter_cap", ctypes.c_uint32),
+                ("max_waves", ctypes.c_uint32), ("waves_per_poll", ctypes.c_uint32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("duration_total_us", ctypes.c_double),
+                ("duration_device_us", ctypes.c_double),
+                ("waves", ctypes.c_uint64), ("tile_visits", ctypes.c_uint64),
+                ("tiles_total", ctypes.c_uint64), ("cells", ctypes.c_uint64),
+                ("inner_iterations", ctypes.c_uint64), ("cells_improved", ctypes.c_uint64),
+                ("converged", ctypes.c_uint32), ("tile_w", ctypes.c_uint32),
+                ("tile_h", ctypes.c_uint32), ("n_active", ctypes.c_uint32),
+                ("bytes_per_visit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
+                ("ordinal", ctypes.c_int32), ("compute_units", ctypes.c_int32),
+                ("clock_khz", ctypes.c_int32), ("max_workgroup", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_uint64), ("global_mem_bytes", ctypes.c_uint64),
+                ("pci_bus_id", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("d_field", ctypes.c_void_p), ("d_time", ctypes.c_void_p),
+                ("w", ctypes.c_uint32), ("h", ctypes.c_uint32),
+                ("seeds_xy", ctypes.c_void_p), ("n_seeds", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+# Every symbol declared by include/ghcm.h and include/eikonal_c.h.
+EXPORTED_SYMBOLS = (
+    "ghcm_abi_version", "ghcm_status_string", "ghcm_last_error", "ghcm_config_default",
+    "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
+    "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
+    "ghcm_solve_batch", "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
+    "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
+)
+
+_lib = None
+
+
+class GhcmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ghcm status {status}: {msg}")
+        self.status = status
+
+
+def lib() -> ctypes.CDLL:
+    """Load libghcm.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    P = ctypes.POINTER
+    L.ghcm_abi_version.restype = i32
+    L.ghcm_status_string.restype = ctypes.c_char_p
+    L.ghcm_status_string.argtypes = [i32]
+    L.ghcm_last_error.restype = ctypes.c_char_p
+    L.ghcm_config_default.argtypes = [P(Config)]
+    L.ghcm_config_default.restype = None
+    L.ghcm_device_count.argtypes = [P(ctypes.c_int)]
+    L.ghcm_device_info_get.argtypes = [i32, P(DeviceInfo)]
+    L.ghcm_ctx_create.argtypes = [i32, P(Config), P(vp)]
+    L.ghcm_ctx_destroy.argtypes = [vp]
+    L.ghcm_ctx_get_config.argtypes = [vp, P(Config)]
+    L.ghcm_ctx_set_config.argtypes = [vp, P(Config)]
+    L.ghcm_solve.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
+    L.ghcm_solve_device.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
+    L.ghcm_solve_batch.argtypes = [vp, i32, P(Instance), u32, P(Stats)]
+    L.ghcm_gen_ring_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_double, vp]
+    L.ghcm_gen_smooth_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, vp]
+    L.ghcm_gen_points.argtypes = [u32, u32, ctypes.c_uint64, u32, vp]
+    L.ghcm_cpu_fmm.argtypes = [vp, u32, u32, vp, u32, vp, P(ctypes.c_double)]
+    L.ghcm_cpu_fim.argtypes = [ctypes.c_float, vp, u32, u32, vp, u32, vp, P(ctypes.c_double)]
+    L.ghcm_hcm_solve_cxx.argtypes = [i32, u32, ctypes.c_float, ctypes.c_float, i32, vp,
+                                     u32, u32, vp, u32, vp, P(Stats)]
+    for name in EXPORTED_SYMBOLS:
+        getattr(L, name).restype = getattr(L, name).restype or i32
+    _lib = L
+    return L
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        L = lib()
+        raise GhcmError(status, (L.ghcm_last_error() or b"").decode() or
+                        L.ghcm_status_string(status).decode())
+
+
+def status_string(status: int) -> str:
+    return lib().ghcm_status_string(status).decode()
+
+
+def default_config(**overrides) -> Config:
+    c = Config()
+    lib().ghcm_config_default(ctypes.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise TypeError(f"unknown config field {k}")
+        setattr(c, k, v)
+    return c
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    st = lib().ghcm_device_count(ctypes.byref(n))
+    return n.value if st == OK else 0
+
+
+def device_info(ordinal: int = 0) -> dict:
+    d = DeviceInfo()
+    _check(lib().ghcm_device_info_get(ordinal, ctypes.byref(d)))
+    return {"name": d.name.decode(), "arch": d.arch.decode(), "ordinal": d.ordinal,
+            "compute_units": d.compute_units, "clock_khz": d.clock_khz,
+            "lds_bytes": d.lds_bytes, "global_mem_bytes": d.global_mem_bytes}
+
+
+def _seeds_array(seeds) -> np.ndarray:
+    s = np.asarray(seeds, dtype=np.uint32).reshape(-1, 2)
+    return np.ascontiguousarray(s)
+
+
+def _field_array(field: np.ndarray, kind: int) -> np.ndarray:
+    if kind in (FIELD_COST_F32, FIELD_SPEED_F32):
+        return np.ascontiguousarray(field, dtype=np.float32)
+    if kind == FIELD_SPEED_F16:
+        return np.ascontiguousarray(field, dtype=np.float16)
+    if kind == FIELD_SPEED_BF16:
+        a = np.ascontiguousarray(field)
+        if a.dtype != np.uint16:
+            raise TypeError("bf16 fields are passed as uint16 bit patterns (see to_bf16)")
+        return a
+    raise ValueError(f"unknown field kind {kind}")
+
+
+def to_bf16(x: np.ndarray) -> np.ndarray:
+    """FP32 -> bfloat16 bit patterns (uint16), round to nearest even."""
+    f = np.ascontiguousarray(x, dtype=np.float32)
+    b = f.view(np.uint32).astype(np.uint64)
+    rounded = (b + 0x7FFF + ((b >> 16) & 1)) >> 16
+    nan = np.isnan(f)
+    out = rounded.astype(np.uint16)
+    out[nan] = 0x7FC0
+    return out
+
+
+def from_bf16(b: np.ndarray) -> np.ndarray:
+    return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32)
+
+
+class Context:
+    """A solver context on one HIP device (HcmSetup of src/eikonal.h:53-63)."""
+
+    def __init__(self, ordinal: int = 0, **config):
+        self._L = lib()
+        self.config = default_config(**config)
+        h = ctypes.c_void_p()
+        _check(self._L.ghcm_ctx_create(ordinal, ctypes.byref(self.config), ctypes.byref(h)))
+        self._h = h
+        self.ordinal = ordinal
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.ghcm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_config(self, **changes) -> None:
+        c = Config()
+        _check(self._L.ghcm_ctx_get_config(self._h, ctypes.byref(c)))
+        for k, v in changes.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown config field {k}")
+            setattr(c, k, v)
+        _check(self._L.ghcm_ctx_set_config(self._h, ctypes.byref(c)))
+        self.config = c
+
+    def solve(self, field: np.ndarray, seeds, kind: int = FIELD_COST_F32,
+              allow_not_converged: bool = False):
+        """Solve one field held in host memory; returns (times (H, W) float32, stats)."""
+        f = _field_array(field, kind)
+        if f.ndim != 2:
+            raise ValueError("field must be 2-D (H, W)")
+        h, w = f.shape
+        s = _seeds_array(seeds)
+        out = np.empty((h, w), dtype=np.float32)
+        st = Stats()
+        rc = self._L.ghcm_solve(self._h, f.ctypes.data, kind, w, h, s.ctypes.data, s.shape[0],
+                                out.ctypes.data, ctypes.byref(st))
+        if not (allow_not_converged and rc == ERR_NOT_CONVERGED):
+            _check(rc)
+        return out, st.to_dict()
+
+    def solve_device(self, d_field: int, kind: int, w: int, h: int, seeds, d_time: int) -> dict:
+        """Solve a field already resident in device memory (raw device pointers)."""
+        s = _seeds_array(seeds)
+        st = Stats()
+        _check(self._L.ghcm_solve_device(self._h, ctypes.c_void_p(d_field), kind, w, h,
+                                         s.ctypes.data, s.shape[0], ctypes.c_void_p(d_time),
+                                         ctypes.byref(st)))
+        return st.to_dict()
+
+    def solve_batch(self, kind: int, instances: Sequence[tuple]) -> dict:
+        """instances: (d_field_ptr, d_time_ptr, w, h, seeds) tuples, device pointers."""
+        n = len(instances)
+        arr = (Instance * n)()
+        keep = []
+        for i, (df, dt, w, h, seeds) in enumerate(instances):
+            s = _seeds_array(seeds)
+            keep.append(s)
+            arr[i] = Instance(df, dt, w, h, s.ctypes.data, s.shape[0], 0)
+        st = Stats()
+        _check(self._L.ghcm_solve_batch(self._h, kind, arr, n, ctypes.byref(st)))
+        return st.to_dict()
+
+
+# ---- CPU solvers of the C++ API (eikonal_fmm_sequential / eikonal_fim_sequential) ----
+
+def fmm_sequential(cost: np.ndarray, seeds):
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    h, w = c.shape
+    s = _seeds_array(seeds)
+    out = np.empty_like(c)
+    us = ctypes.c_double(0)
+    _check(lib().ghcm_cpu_fmm(c.ctypes.data, w, h, s.ctypes.data, s.shape[0],
+                              out.ctypes.data, ctypes.byref(us)))
+    return out, us.value
+
+
+def fim_sequential(minimum_change: float, cost: np.ndarray, seeds):
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    h, w = c.shape
+    s = _seeds_array(seeds)
+    out = np.empty_like(c)
+    us = ctypes.c_double(0)
+    _check(lib().ghcm_cpu_fim(minimum_change, c.ctypes.data, w, h, s.ctypes.data, s.shape[0],
+                              out.ctypes.data, ctypes.byref(us)))
+    return out, us.value
+
+
+def hcm_solve_cxx(cost: np.ndarray, seeds, block_size: int = 8, compute_unit_overload: float = 1.0,
+                  fim_minimum_change: float = 1e-5, kernel: int = KERNEL_SYSTOLIC,
+                  ordinal: int = 0):
+    """eikonal_hcm_hip_setup + eikonal_hcm_hip_ex through the C++ layer."""
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    h, w = c.shape
+    s = _seeds_array(seeds)
+    out = np.empty_like(c)
+    st = Stats()
+    _check(lib().ghcm_hcm_solve_cxx(ordinal, block_size, compute_unit_overload,
+                                    fim_minimum_change, kernel, c.ctypes.data, w, h,
+                                    s.ctypes.data, s.shape[0], out.ctypes.data,
+                                    ctypes.byref(st)))
+    return out, st.to_dict()
+
+
+# ---- synthetic workloads ----
+
+def gen_ring_speed(w: int, h: int, seed: int, wall_speed: float = 1e-3,
+                   scale: float = 1.0) -> np.ndarray:
+    out = np.empty((h, w), dtype=np.float64)
+    _check(lib().ghcm_gen_ring_speed(w, h, seed, wall_speed, scale, out.ctypes.data))
+    return out
+
+
+def gen_smooth_speed(w: int, h: int, seed: int, sigma: float = 32.0, lo: float = 0.1,
+                     hi: float = 10.0) -> np.ndarray:
+    out = np.empty((h, w), dtype=np.float64)
+    _check(lib().ghcm_gen_smooth_speed(w, h, seed, sigma, lo, hi, out.ctypes.data))
+    return out
+
+
+def gen_points(w: int, h: int, seed: int, n: int) -> np.ndarray:
+    out = np.empty((n, 2), dtype=np.uint32)
+    _check(lib().ghcm_gen_points(w, h, seed, n, out.ctypes.data))
+    return out
+
+
+def cost_from_speed(speed: np.ndarray) -> np.ndarray:
+    """cost = 1/speed in FP32 (src/test_runner.cpp:32-36); speed 0 -> +INF."""
+    with np.errstate(divide="ignore"):
+        return (np.float32(1.0) / np.asarray(speed, dtype=np.float32)).astype(np.float32)

@@ -1,0 +1,603 @@
+// ghcm_capi.cpp -- C-ABI of libghcm: devices, contexts, and the two wave schedulers.
+//
+// Replaces src/util_opencl.{h,cpp} (device enumeration/info, error strings, cl_do) and the
+// host half of eikonal_hcm_opencl_ex (src/eikonal_hcm.cpp:186-529).  Nothing here exits
+// or throws across the C boundary: every entry point returns a ghcm_status and records a
+// message retrievable with ghcm_last_error().
+#include "ghcm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <queue>
+#include <string>
+#include <vector>
+
+#include "ghcm_dev.h"
+#include "kernels.h"
+
+using namespace ghcm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(GHCM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+// Device buffer that only grows; reused across solves on one context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want) {
+        if (want <= bytes) return GHCM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(want, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        }
+        bytes = b;
+        return GHCM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want) {
+        if (want <= bytes) return GHCM_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(want, 256);
+        hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        }
+        bytes = b;
+        return GHCM_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+bool valid_kind(int k) { return k >= GHCM_FIELD_COST_F32 && k <= GHCM_FIELD_SPEED_BF16; }
+size_t kind_bytes(int k) { return (k == GHCM_FIELD_SPEED_F16 || k == GHCM_FIELD_SPEED_BF16) ? 2 : 4; }
+
+}  // namespace
+
+struct ghcm_ctx {
+    int ordinal = 0;
+    ghcm_config cfg{};
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_poll[2] = {nullptr, nullptr};
+    // device scratch
+    DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status;
+    DevBuf field_io, time_io;   // staging for the host-buffer entry point
+    PinnedBuf h_ctrl, h_init, h_seeds, h_inst, h_jobs, h_status;
+};
+
+// ---------------------------------------------------------------------------------------
+// library / device
+// ---------------------------------------------------------------------------------------
+
+extern "C" int ghcm_abi_version(void) { return GHCM_ABI_VERSION; }
+
+extern "C" const char* ghcm_status_string(int s) {
+    switch (s) {
+        case GHCM_OK: return "ok";
+        case GHCM_ERR_INVALID_ARG: return "invalid argument";
+        case GHCM_ERR_NO_DEVICE: return "no such device";
+        case GHCM_ERR_HIP: return "HIP runtime error";
+        case GHCM_ERR_OUT_OF_MEMORY: return "out of memory";
+        case GHCM_ERR_NOT_CONVERGED: return "did not converge within the wave cap";
+        case GHCM_ERR_UNSUPPORTED: return "no kernel built for this configuration";
+        case GHCM_ERR_OVER_BUDGET: return "request exceeds the HBM budget";
+        case GHCM_ERR_NOT_FOUND: return "unknown ticket";
+        case GHCM_ERR_STOPPED: return "server stopped";
+    }
+    return "unknown status";
+}
+
+extern "C" const char* ghcm_last_error(void) { return g_last_error.c_str(); }
+
+// HcmConfig defaults (src/eikonal.h:12-27): block 8, overloads 1, epsilon 1e-5,
+// cache Both, Systolic.
+extern "C" void ghcm_config_default(ghcm_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->tile_w = 8;
+    c->tile_h = 8;
+    c->workers_overload = 1.0f;
+    c->compute_unit_overload = 1.0f;
+    c->n_active = 0;
+    c->min_change = 1e-5f;
+    c->cache_mode = GHCM_CACHE_BOTH;
+    c->kernel = GHCM_KERNEL_SYSTOLIC;
+    c->scheduler = GHCM_SCHED_DEVICE_TOPN;
+    c->inner_iter_cap = 0;
+    c->max_waves = 0;
+    c->waves_per_poll = 0;
+}
+
+extern "C" int ghcm_device_count(int* count) {
+    if (!count) return fail(GHCM_ERR_INVALID_ARG, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(GHCM_ERR_NO_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_device_info_get(int ordinal, ghcm_device_info* out) {
+    if (!out) return fail(GHCM_ERR_INVALID_ARG, "out is null");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || ordinal < 0 || ordinal >= n)
+        return fail(GHCM_ERR_NO_DEVICE, "device %d not present (%d devices)", ordinal, n);
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, ordinal));
+    std::memset(out, 0, sizeof *out);
+    std::snprintf(out->name, sizeof out->name, "%s", p.name);
+    std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    out->ordinal = ordinal;
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    out->max_workgroup = p.maxThreadsPerBlock;
+    out->lds_bytes = p.sharedMemPerBlock;
+    out->global_mem_bytes = p.totalGlobalMem;
+    out->pci_bus_id = p.pciBusID;
+    return GHCM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------
+
+static int check_config(const ghcm_config* c) {
+    if (!c) return fail(GHCM_ERR_INVALID_ARG, "config is null");
+    if (!(c->min_change >= 0.0f) || !std::isfinite(c->min_change))
+        return fail(GHCM_ERR_INVALID_ARG, "min_change must be finite and >= 0");
+    if (c->kernel != GHCM_KERNEL_SYSTOLIC && c->kernel != GHCM_KERNEL_FIM)
+        return fail(GHCM_ERR_INVALID_ARG, "unknown kernel %d", c->kernel);
+    if (c->scheduler != GHCM_SCHED_DEVICE_TOPN && c->scheduler != GHCM_SCHED_HOST_PQ)
+        return fail(GHCM_ERR_INVALID_ARG, "unknown scheduler %d", c->scheduler);
+    if (c->cache_mode < GHCM_CACHE_NONE || c->cache_mode > GHCM_CACHE_BOTH)
+        return fail(GHCM_ERR_INVALID_ARG, "unknown cache mode %d", c->cache_mode);
+    if (!(c->compute_unit_overload > 0.0f) || !(c->workers_overload > 0.0f))
+        return fail(GHCM_ERR_INVALID_ARG, "overloads must be > 0");
+    for (int k = 0; k < 4; ++k)
+        if (!get_solve_launcher((int)c->tile_w, (int)c->tile_h, k, c->kernel, false))
+            return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u", c->tile_w, c->tile_h);
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** out) {
+    if (!out) return fail(GHCM_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    ghcm_config c;
+    if (cfg) c = *cfg; else ghcm_config_default(&c);
+    int rc = check_config(&c);
+    if (rc) return rc;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || ordinal < 0 || ordinal >= n)
+        return fail(GHCM_ERR_NO_DEVICE, "device %d not present (%d devices)", ordinal, n);
+    HIP_TRY(hipSetDevice(ordinal));
+    ghcm_ctx* ctx = new (std::nothrow) ghcm_ctx();
+    if (!ctx) return fail(GHCM_ERR_OUT_OF_MEMORY, "context allocation failed");
+    ctx->ordinal = ordinal;
+    ctx->cfg = c;
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDeviceProperties(&p, ordinal);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_begin);
+    if (e == hipSuccess) e = hipEventCreate(&ctx->ev_end);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        ghcm_ctx_destroy(ctx);
+        return fail(GHCM_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
+    }
+    ctx->cus = p.multiProcessorCount;
+    *out = ctx;
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
+    if (!ctx) return GHCM_OK;
+    (void)hipSetDevice(ctx->ordinal);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : {&ctx->inst, &ctx->tinst, &ctx->qwave, &ctx->key, &ctx->mask, &ctx->tmin,
+                      &ctx->list, &ctx->ctrl, &ctx->seeds, &ctx->jobs, &ctx->status,
+                      &ctx->field_io, &ctx->time_io})
+        b->release();
+    for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_init, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
+                         &ctx->h_status})
+        b->release();
+    if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
+    if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
+    for (hipEvent_t ev : ctx->ev_poll)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_ctx_get_config(const ghcm_ctx* ctx, ghcm_config* out) {
+    if (!ctx || !out) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    *out = ctx->cfg;
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_ctx_set_config(ghcm_ctx* ctx, const ghcm_config* cfg) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    ctx->cfg = *cfg;
+    return GHCM_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// shared solve core
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+struct HostInst {      // host view of one instance
+    const void* d_field;
+    float* d_time;
+    uint32_t w, h;
+    const uint32_t* seeds;
+    uint32_t n_seeds;
+};
+
+uint32_t bytes_per_visit(uint32_t tw, uint32_t th, int kind) {
+    // speed/cost read + time read (tile + 4 halo edges) + time write + 40 B metadata.
+    uint32_t P = tw * th;
+    return P * (uint32_t)kind_bytes(kind) + (P + 2 * (tw + th)) * 4 + P * 4 + 40;
+}
+
+int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats) {
+    const auto t_wall0 = std::chrono::steady_clock::now();
+    const ghcm_config& cfg = ctx->cfg;
+    if (!valid_kind(kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", kind);
+    if (n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
+    const uint32_t tw = cfg.tile_w, th = cfg.tile_h;
+    const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
+    solve_launcher_t launch = get_solve_launcher((int)tw, (int)th, kind, cfg.kernel, host_sched);
+    if (!launch)
+        return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u kind %d kernel %d%s", tw, th,
+                    kind, cfg.kernel, host_sched ? " (host scheduler)" : "");
+
+    // ---- instance table, seeds ----
+    int rc = ctx->h_inst.reserve(sizeof(GhcmInst) * n_inst);
+    if (rc) return rc;
+    GhcmInst* hinst = ctx->h_inst.as<GhcmInst>();
+    uint64_t tiles64 = 0, cells = 0, n_seeds_total = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+        const HostInst& x = hi[i];
+        if (x.w == 0 || x.h == 0) return fail(GHCM_ERR_INVALID_ARG, "instance %u has zero size", i);
+        if (!x.d_field || !x.d_time) return fail(GHCM_ERR_INVALID_ARG, "instance %u has a null buffer", i);
+        if (x.n_seeds && !x.seeds) return fail(GHCM_ERR_INVALID_ARG, "instance %u seeds are null", i);
+        for (uint32_t s = 0; s < x.n_seeds; ++s)
+            if (x.seeds[2 * s] >= x.w || x.seeds[2 * s + 1] >= x.h)
+                return fail(GHCM_ERR_INVALID_ARG, "instance %u seed %u (%u,%u) outside %ux%u", i, s,
+                            x.seeds[2 * s], x.seeds[2 * s + 1], x.w, x.h);
+        GhcmInst& g = hinst[i];
+        g.field = x.d_field;
+        g.time = x.d_time;
+        g.w = x.w;
+        g.h = x.h;
+        g.tiles_x = (x.w + tw - 1) / tw;
+        g.tiles_y = (x.h + th - 1) / th;
+        g.tile_base = (uint32_t)tiles64;
+        g.pad = 0;
+        tiles64 += (uint64_t)g.tiles_x * g.tiles_y;
+        cells += (uint64_t)x.w * x.h;
+        n_seeds_total += x.n_seeds;
+    }
+    if (tiles64 >= 0x7FFFFFFFull) return fail(GHCM_ERR_INVALID_ARG, "too many tiles");
+    const uint32_t T = (uint32_t)tiles64;
+
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    hipStream_t st = ctx->stream;
+    if ((rc = ctx->inst.reserve(sizeof(GhcmInst) * n_inst))) return rc;
+    if ((rc = ctx->qwave.reserve(4ull * T))) return rc;
+    if ((rc = ctx->key.reserve(8ull * T))) return rc;
+    if ((rc = ctx->mask.reserve(8ull * T))) return rc;
+    if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
+    if ((rc = ctx->list.reserve(8ull * T))) return rc;
+    if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
+    if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
+    if ((rc = ctx->seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
+    if ((rc = ctx->h_seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
+    if ((rc = ctx->h_init.reserve(sizeof(GhcmCtrl)))) return rc;
+    if ((rc = ctx->h_ctrl.reserve(2 * sizeof(GhcmCtrl)))) return rc;
+
+    uint32_t* hs = ctx->h_seeds.as<uint32_t>();
+    {
+        size_t k = 0;
+        for (uint32_t i = 0; i < n_inst; ++i)
+            for (uint32_t s = 0; s < hi[i].n_seeds; ++s) {
+                hs[k++] = hi[i].seeds[2 * s];
+                hs[k++] = hi[i].seeds[2 * s + 1];
+                hs[k++] = i;
+            }
+    }
+    uint32_t n_sel = cfg.n_active ? cfg.n_active
+                                  : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * ctx->cus));
+    GhcmCtrl* init = ctx->h_init.as<GhcmCtrl>();
+    std::memset(init, 0, sizeof *init);
+    init->n_select = n_sel;
+
+    SolveArgs a{};
+    a.inst = ctx->inst.as<GhcmInst>();
+    a.n_inst = n_inst;
+    a.tinst = n_inst > 1 ? ctx->tinst.as<uint32_t>() : nullptr;
+    a.qwave = ctx->qwave.as<uint32_t>();
+    a.key = ctx->key.as<uint32_t>();
+    a.mask = ctx->mask.as<uint32_t>();
+    a.tmin = ctx->tmin.as<float>();
+    a.list = ctx->list.as<uint32_t>();
+    a.tiles = T;
+    a.ctrl = ctx->ctrl.as<GhcmCtrl>();
+    a.eps = cfg.min_change;
+    a.iter_cap = cfg.inner_iter_cap;
+
+    HIP_TRY(hipEventRecord(ctx->ev_begin, st));
+    HIP_TRY(hipMemcpyAsync(a.ctrl, init, sizeof(GhcmCtrl), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->inst.p, hinst, sizeof(GhcmInst) * n_inst, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->seeds.p, hs, 12ull * n_seeds_total, hipMemcpyHostToDevice, st));
+    for (uint32_t i = 0; i < n_inst; ++i)
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(hi[i].d_time), (size_t)hi[i].w * hi[i].h, kInfBits, st));
+    HIP_TRY(launch_fill_u32(a.qwave, T, 0u, st));
+    HIP_TRY(launch_fill_u32(a.key, 2ull * T, kInfBits, st));
+    HIP_TRY(launch_fill_u32(a.mask, 2ull * T, 0u, st));
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a.tmin), T, kInfBits, st));
+    if (n_inst > 1) HIP_TRY(launch_tile_inst(ctx->tinst.as<uint32_t>(), a.inst, n_inst, st));
+    HIP_TRY(launch_seed(a, ctx->seeds.as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
+
+    uint64_t max_waves = cfg.max_waves ? cfg.max_waves : (uint64_t)8 * T + 100000;
+    bool converged = false;
+    uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0;
+
+    if (!host_sched) {
+        // ---- device-resident wave loop; the host only polls a done flag ----
+        HIP_TRY(launch_wave_end(a, 1u, st));
+        GhcmCtrl* hc = ctx->h_ctrl.as<GhcmCtrl>();
+        uint32_t chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
+        const uint32_t chunk_max = cfg.waves_per_poll ? cfg.waves_per_poll : 64;
+        uint64_t enqueued = 0;
+        int inflight = 0, slot = 0;
+        // Two chunks in flight: the GPU never waits on the host's poll.
+        for (;;) {
+            for (uint32_t k = 0; k < chunk; ++k) {
+                HIP_TRY(launch(a, n_sel, st));
+                HIP_TRY(launch_wave_end(a, 0u, st));
+            }
+            enqueued += chunk;
+            HIP_TRY(hipMemcpyAsync(&hc[slot], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(ctx->ev_poll[slot], st));
+            ++inflight;
+            slot ^= 1;
+            if (inflight == 2) {
+                HIP_TRY(hipEventSynchronize(ctx->ev_poll[slot]));
+                --inflight;
+                if (hc[slot].done) { converged = true; break; }
+            }
+            if (enqueued >= max_waves) break;
+            if (chunk < chunk_max) chunk *= 2;
+        }
+        HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(ctx->ev_end, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        converged = hc[0].done != 0;
+        waves_done = hc[0].busy_waves;
+        visits = hc[0].visits;
+        inner = hc[0].inner_iters;
+        improved = hc[0].improved;
+    } else {
+        // ---- host priority queue (src/eikonal_hcm.cpp:343-482) ----
+        if ((rc = ctx->jobs.reserve(sizeof(HostJob) * n_sel))) return rc;
+        if ((rc = ctx->status.reserve(sizeof(HostStatus) * n_sel))) return rc;
+        if ((rc = ctx->h_jobs.reserve(sizeof(HostJob) * n_sel))) return rc;
+        if ((rc = ctx->h_status.reserve(sizeof(HostStatus) * n_sel))) return rc;
+        a.jobs = ctx->jobs.as<HostJob>();
+        a.status = ctx->status.as<HostStatus>();
+        HostJob* hj = ctx->h_jobs.as<HostJob>();
+        HostStatus* hst = ctx->h_status.as<HostStatus>();
+        std::vector<float> smallest(T, INFINITY);
+        std::vector<uint32_t> border(T, 0);
+        std::vector<uint8_t> dirty(T, 0);
+        std::vector<uint32_t> inst_of(T);
+        for (uint32_t i = 0; i < n_inst; ++i)
+            std::fill(inst_of.begin() + hinst[i].tile_base,
+                      inst_of.begin() + hinst[i].tile_base + hinst[i].tiles_x * hinst[i].tiles_y, i);
+        auto nbr = [&](uint32_t id, int s, uint32_t* out) -> bool {
+            const GhcmInst& g = hinst[inst_of[id]];
+            uint32_t lt = id - g.tile_base;
+            int x = (int)(lt % g.tiles_x) + (s == 0 ? -1 : s == 2 ? 1 : 0);
+            int y = (int)(lt / g.tiles_x) + (s == 1 ? -1 : s == 3 ? 1 : 0);
+            if (x < 0 || y < 0 || x >= (int)g.tiles_x || y >= (int)g.tiles_y) return false;
+            *out = g.tile_base + (uint32_t)y * g.tiles_x + (uint32_t)x;
+            return true;
+        };
+        // BlockWrapper key: min smallest_time of the tile and its 4 neighbours, frozen
+        // at push (src/eikonal_hcm.cpp:47-80).
+        auto key_of = [&](uint32_t id) {
+            float k = smallest[id];
+            uint32_t nb;
+            for (int s = 0; s < 4; ++s)
+                if (nbr(id, s, &nb)) k = std::min(k, smallest[nb]);
+            return k;
+        };
+        struct Ent {
+            float key;
+            uint32_t id;
+            bool operator<(const Ent& o) const { return key > o.key; }
+        };
+        std::priority_queue<Ent> pq;
+        for (uint32_t i = 0; i < n_inst; ++i)
+            for (uint32_t s = 0; s < hi[i].n_seeds; ++s) {
+                const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
+                for (int d = 0; d < 5; ++d) {
+                    int px = (int)hi[i].seeds[2 * s] + dx[d], py = (int)hi[i].seeds[2 * s + 1] + dy[d];
+                    if (px < 0 || py < 0 || px >= (int)hi[i].w || py >= (int)hi[i].h) continue;
+                    uint32_t t = hinst[i].tile_base + (uint32_t)(py / (int)th) * hinst[i].tiles_x + (uint32_t)(px / (int)tw);
+                    border[t] |= kMaskOpenAll;
+                    smallest[t] = 0.0f;
+                    if (!dirty[t]) { dirty[t] = 1; pq.push({0.0f, t}); }
+                }
+            }
+        while (!pq.empty()) {
+            if (waves_done >= max_waves) break;
+            uint32_t nj = 0;
+            while (nj < n_sel && !pq.empty()) {
+                uint32_t id = pq.top().id;
+                pq.pop();
+                hj[nj].id = id;
+                hj[nj].border = border[id];
+                border[id] = 0;
+                dirty[id] = 0;   // cleared before results, so same-wave neighbours re-queue
+                ++nj;
+            }
+            a.n_jobs = nj;
+            HIP_TRY(hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * nj, hipMemcpyHostToDevice, st));
+            HIP_TRY(launch(a, nj, st));
+            HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            ++waves_done;
+            visits += nj;
+            for (uint32_t j = 0; j < nj; ++j) {
+                const uint32_t id = hj[j].id;
+                const HostStatus& s = hst[j];
+                smallest[id] = s.smallest_time;
+                inner += s.solves;
+                improved += s.cells_updated;
+                for (int side = 0; side < 4; ++side) {
+                    uint32_t nb;
+                    if (!s.border[side] || !nbr(id, side, &nb)) continue;
+                    border[nb] |= 1u << (side ^ 2);
+                    if (!dirty[nb]) { dirty[nb] = 1; pq.push({key_of(nb), nb}); }
+                }
+                if (s.requeue) {
+                    border[id] |= kMaskOpenAll;
+                    if (!dirty[id]) { dirty[id] = 1; pq.push({key_of(id), id}); }
+                }
+            }
+        }
+        converged = pq.empty();
+        HIP_TRY(hipEventRecord(ctx->ev_end, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+
+    float dev_ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&dev_ms, ctx->ev_begin, ctx->ev_end));
+    const auto t_wall1 = std::chrono::steady_clock::now();
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->duration_total_us = std::chrono::duration<double, std::micro>(t_wall1 - t_wall0).count();
+        stats->duration_device_us = 1000.0 * dev_ms;
+        stats->waves = waves_done;
+        stats->tile_visits = visits;
+        stats->tiles_total = T;
+        stats->cells = cells;
+        stats->inner_iterations = inner;
+        stats->cells_improved = improved;
+        stats->converged = converged ? 1u : 0u;
+        stats->tile_w = tw;
+        stats->tile_h = th;
+        stats->n_active = n_sel;
+        stats->bytes_per_visit = bytes_per_visit(tw, th, kind);
+    }
+    if (!converged) return fail(GHCM_ERR_NOT_CONVERGED, "wave cap %llu reached", (unsigned long long)max_waves);
+    return GHCM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// solve entry points
+// ---------------------------------------------------------------------------------------
+
+extern "C" int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
+                                 uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
+                                 float* d_time, ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    HostInst hi{d_field, d_time, w, h, seeds_xy, n_seeds};
+    return solve_core(ctx, field_kind, &hi, 1, stats);
+}
+
+extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
+                          const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
+                          ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    if (!field || !time_out) return fail(GHCM_ERR_INVALID_ARG, "null field or output");
+    if (!valid_kind(field_kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", field_kind);
+    if (w == 0 || h == 0) return fail(GHCM_ERR_INVALID_ARG, "zero-sized field");
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    const size_t n = (size_t)w * h;
+    int rc = ctx->field_io.reserve(n * kind_bytes(field_kind));
+    if (rc) return rc;
+    if ((rc = ctx->time_io.reserve(n * 4))) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->field_io.p, field, n * kind_bytes(field_kind), hipMemcpyHostToDevice, ctx->stream));
+    HostInst hi{ctx->field_io.p, ctx->time_io.as<float>(), w, h, seeds_xy, n_seeds};
+    rc = solve_core(ctx, field_kind, &hi, 1, stats);
+    if (rc != GHCM_OK && rc != GHCM_ERR_NOT_CONVERGED) return rc;
+    HIP_TRY(hipMemcpyAsync(time_out, ctx->time_io.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->duration_total_us =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+extern "C" int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst,
+                                uint32_t n_inst, ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    if (!inst || n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
+    std::vector<HostInst> hi(n_inst);
+    for (uint32_t i = 0; i < n_inst; ++i)
+        hi[i] = HostInst{inst[i].d_field, inst[i].d_time, inst[i].w, inst[i].h, inst[i].seeds_xy, inst[i].n_seeds};
+    return solve_core(ctx, field_kind, hi.data(), n_inst, stats);
+}

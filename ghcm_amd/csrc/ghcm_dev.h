@@ -1,0 +1,90 @@
+// ghcm_dev.h -- structures shared by the host scheduler and the gfx950 kernels.
+//
+// HBM layout of one solve (or one batch of instances):
+//   field[W*H]       per instance, COST_F32 / SPEED_{F32,F16,BF16}, row-major
+//   time [W*H]       per instance, FP32, row-major
+//   tile arrays, one entry per tile of every instance (global tile id g):
+//     qwave[g]       u32  wave+1 for which g is queued (0 = never); monotone, so a
+//                         tile is appended at most once per wave list (atomicMax)
+//     key[2][g]      u32  float bits of the smallest candidate time that activated
+//                         g; double-buffered by wave parity
+//     mask[2][g]     u32  border bits (1 << side) + OPEN_ALL; double-buffered
+//     tmin[g]        f32  smallest own-cell time after the last visit
+//     tinst[g]       u32  instance index (batches only)
+//   list[2][tiles]   u32  wave lists; list[cur] is read by wave w, list[cur^1]
+//                         collects activations for wave w+1
+//   ctrl             GhcmCtrl, one per solve
+#pragma once
+
+#include <stdint.h>
+
+namespace ghcm {
+
+constexpr uint32_t kInfBits   = 0x7F800000u;  // +INF as uint32 bits
+constexpr uint32_t kMaskOpenAll = 1u << 4;    // open every cell on the next visit
+// Side numbering of the reference (src/eikonal_hcm.cpp:18, data/eikonal_common.cl:361):
+// 0: x-, 1: y-, 2: x+, 3: y+ ; opposite(s) = s ^ 2.
+
+struct GhcmInst {
+    const void* field;
+    float*      time;
+    uint32_t    w, h;
+    uint32_t    tiles_x, tiles_y;
+    uint32_t    tile_base;
+    uint32_t    pad;
+};
+
+struct GhcmCtrl {
+    uint32_t wave;          // index of the wave being (or about to be) processed
+    uint32_t cur;           // list[cur] holds the tiles queued for `wave`
+    uint32_t count[2];      // list lengths
+    uint32_t done;          // 1 once a wave starts with an empty list
+    uint32_t thr_lo;        // keys < thr_lo are selected
+    uint32_t thr_hi;        // keys in [thr_lo, thr_hi) are selected up to `quota`
+    uint32_t quota;
+    uint32_t quota_used;
+    uint32_t n_select;      // n
+    uint32_t busy_waves;    // waves that had work
+    uint32_t pad0;
+    unsigned long long visits;
+    unsigned long long inner_iters;
+    unsigned long long improved;
+    unsigned long long carried;
+};
+
+// Host-scheduled wave: job and status records (src/eikonal_hcm.cpp:22-32,
+// data/eikonal_common.cl:34-44).  Status carries the smallest time computed by
+// the visit (own cells and border solves) and one dirty flag per side.
+struct HostJob {
+    uint32_t id;
+    uint32_t border;        // 1 << side, plus kMaskOpenAll for seed tiles
+};
+
+struct HostStatus {
+    uint32_t solves;        // in-tile passes
+    uint32_t cells_updated;
+    float    smallest_time;
+    uint32_t border[4];
+    uint32_t requeue;       // 1 if the visit hit the pass cap before converging
+};
+
+struct SolveArgs {
+    const GhcmInst* inst;
+    uint32_t        n_inst;
+    const uint32_t* tinst;
+    uint32_t*       qwave;
+    uint32_t*       key;     // [2][tiles]
+    uint32_t*       mask;    // [2][tiles]
+    float*          tmin;
+    uint32_t*       list;    // [2][tiles]
+    uint32_t        tiles;
+    GhcmCtrl*       ctrl;
+    float           eps;
+    uint32_t        iter_cap;
+    // host-scheduled mode
+    const HostJob*  jobs;
+    HostStatus*     status;
+    uint32_t        n_jobs;
+};
+
+}  // namespace ghcm

@@ -1,0 +1,23 @@
+// kernels.h -- host-side entry points of kernels.hip (launch wrappers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ghcm_dev.h"
+
+namespace ghcm {
+
+using solve_launcher_t = hipError_t (*)(const SolveArgs&, uint32_t grid, hipStream_t);
+
+// Returns nullptr when no kernel is built for (tile_w, tile_h, field kind, mode).
+// mode: 0 systolic, 1 FIM.  host: job list given by the host scheduler.
+solve_launcher_t get_solve_launcher(int tile_w, int tile_h, int kind, int mode, bool host);
+
+hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s);
+hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
+                       uint32_t tile_w, uint32_t tile_h, hipStream_t s);
+
+}  // namespace ghcm

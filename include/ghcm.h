@@ -1,0 +1,175 @@
+/*
+ * ghcm.h -- C-ABI of the MI355X-native gHCM eikonal solver (libghcm.so).
+ *
+ * This layer replaces the reference's OpenCL platform layer (src/util_opencl.h:1-57)
+ * and the device half of src/eikonal_hcm.cpp.  Every function is `extern "C"`,
+ * takes plain pointers and sizes, never exits the process and never throws; it
+ * returns a ghcm_status and leaves a readable message in ghcm_last_error().
+ *
+ * Field layout (as the reference, src/util_dim.h:19-22 and src/util_field.h):
+ * W x H cells, row-major, idx = x + y*W.  A field is either a per-cell COST
+ * (the reference's input, src/eikonal.h:40-67; cost = 1/speed with h = 1,
+ * src/test_runner.cpp:32-36) or a per-cell SPEED (the reduced-precision storage
+ * extension; cost = 1/speed is formed in FP32 on load).  Times are FP32;
+ * unreached cells and obstacles (cost = +INF, speed = 0) stay +INF.
+ * Seeds are (x, y) pairs of uint32; T(seed) = 0.
+ */
+#ifndef GHCM_H
+#define GHCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GHCM_ABI_VERSION 1
+
+/* Status codes.  Replaces cl::Error + cl_do's exit(-1) (src/util_opencl.h:46-56). */
+typedef enum ghcm_status {
+    GHCM_OK                  = 0,
+    GHCM_ERR_INVALID_ARG     = 1,  /* bad size, null pointer, seed out of grid, bad enum */
+    GHCM_ERR_NO_DEVICE       = 2,  /* ordinal out of range / no HIP device              */
+    GHCM_ERR_HIP             = 3,  /* a HIP runtime call failed                          */
+    GHCM_ERR_OUT_OF_MEMORY   = 4,  /* device or host allocation failed                   */
+    GHCM_ERR_NOT_CONVERGED   = 5,  /* wave cap reached with tiles still queued           */
+    GHCM_ERR_UNSUPPORTED     = 6,  /* valid enum, but no kernel built for that combination */
+    GHCM_ERR_OVER_BUDGET     = 7,  /* serving: request larger than the HBM budget         */
+    GHCM_ERR_NOT_FOUND       = 8,  /* serving: unknown ticket                             */
+    GHCM_ERR_STOPPED         = 9   /* serving: server stopped                             */
+} ghcm_status;
+
+/* In-tile solver style (HcmConfig::Kernel, src/eikonal.h:13). */
+typedef enum ghcm_kernel {
+    GHCM_KERNEL_SYSTOLIC = 0,      /* data/eikonal_ghcm_systolic.cl */
+    GHCM_KERNEL_FIM      = 1       /* data/eikonal_ghcm_fim.cl      */
+} ghcm_kernel;
+
+/* Who picks the n lowest-min-time tiles of a wave. */
+typedef enum ghcm_scheduler {
+    GHCM_SCHED_DEVICE_TOPN = 0,    /* device-resident wave loop with device top-n      */
+    GHCM_SCHED_HOST_PQ     = 1     /* host std::priority_queue, src/eikonal_hcm.cpp:343-482 */
+} ghcm_scheduler;
+
+/* HcmConfig::CacheMode (src/eikonal.h:14).  Accepted for source compatibility:
+ * tiles are always staged in LDS with their halo, so every mode behaves as BOTH. */
+typedef enum ghcm_cache_mode {
+    GHCM_CACHE_NONE = 0, GHCM_CACHE_TIME = 1, GHCM_CACHE_COST = 2, GHCM_CACHE_BOTH = 3
+} ghcm_cache_mode;
+
+/* Storage of the input field. */
+typedef enum ghcm_field_kind {
+    GHCM_FIELD_COST_F32   = 0,     /* reference input: FP32 cost                  */
+    GHCM_FIELD_SPEED_F32  = 1,     /* FP32 speed, cost = 1/speed in FP32          */
+    GHCM_FIELD_SPEED_F16  = 2,     /* IEEE half speed                             */
+    GHCM_FIELD_SPEED_BF16 = 3      /* bfloat16 speed                              */
+} ghcm_field_kind;
+
+/* Solver configuration.  Mirrors HcmConfig (src/eikonal.h:12-27) plus the
+ * MI355X-specific knobs.  Fill with ghcm_config_default() first. */
+typedef struct ghcm_config {
+    uint32_t tile_w, tile_h;        /* tile shape; block_size x block_size in the reference.
+                                       Built shapes: 8x8, 16x16, 32x8, 32x32          */
+    float    workers_overload;      /* reference knob (WG = overload x 64); one wave64
+                                       solves a tile here, so it is recorded only     */
+    float    compute_unit_overload; /* n = max(1, overload x #CUs) tiles per wave     */
+    uint32_t n_active;              /* if non-zero, overrides n                       */
+    float    min_change;            /* fim_minimum_change (epsilon), >= 0             */
+    int32_t  cache_mode;            /* ghcm_cache_mode (no effect, see above)         */
+    int32_t  kernel;                /* ghcm_kernel                                    */
+    int32_t  scheduler;             /* ghcm_scheduler                                 */
+    uint32_t inner_iter_cap;        /* in-tile passes per visit, 0 = until converged  */
+    uint32_t max_waves;             /* safety cap on waves, 0 = automatic             */
+    uint32_t waves_per_poll;        /* device scheduler: waves enqueued between host
+                                       polls of the done flag, 0 = automatic          */
+} ghcm_config;
+
+/* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */
+typedef struct ghcm_stats {
+    double   duration_total_us;     /* wall time of the solve call                    */
+    double   duration_device_us;    /* GPU time between the first and last event      */
+    uint64_t waves;                 /* waves that processed at least one tile         */
+    uint64_t tile_visits;           /* tiles solved (sum over waves)                  */
+    uint64_t tiles_total;           /* tiles in the grid(s)                           */
+    uint64_t cells;                 /* W*H summed over instances                      */
+    uint64_t inner_iterations;      /* in-tile passes summed over visits              */
+    uint64_t cells_improved;        /* cell commits that lowered a time               */
+    uint32_t converged;             /* 1 if the queue drained                         */
+    uint32_t tile_w, tile_h;        /* tile shape used                                */
+    uint32_t n_active;              /* n used                                         */
+    uint32_t bytes_per_visit;       /* algorithmic HBM bytes of one tile visit        */
+    uint32_t reserved;
+} ghcm_stats;
+
+/* Device description (OclDevInfo fields, src/util_opencl.h:4-16). */
+typedef struct ghcm_device_info {
+    char     name[256];
+    char     arch[64];              /* e.g. "gfx950"                                  */
+    int32_t  ordinal;
+    int32_t  compute_units;
+    int32_t  clock_khz;
+    int32_t  max_workgroup;
+    uint64_t lds_bytes;             /* per workgroup                                  */
+    uint64_t global_mem_bytes;
+    int32_t  pci_bus_id;
+    int32_t  reserved;
+} ghcm_device_info;
+
+typedef struct ghcm_ctx ghcm_ctx;   /* owns a HIP stream and scratch; one host thread at a time */
+
+/* ---- library / device ---------------------------------------------------- */
+int         ghcm_abi_version(void);
+const char* ghcm_status_string(int status);
+const char* ghcm_last_error(void);  /* thread-local message of the last failure   */
+void        ghcm_config_default(ghcm_config* cfg);
+int         ghcm_device_count(int* count);
+int         ghcm_device_info_get(int ordinal, ghcm_device_info* out);
+
+/* ---- context (HcmSetup, src/eikonal.h:53-63) ------------------------------ */
+int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** out);
+int ghcm_ctx_destroy(ghcm_ctx* ctx);
+int ghcm_ctx_get_config(const ghcm_ctx* ctx, ghcm_config* out);
+int ghcm_ctx_set_config(ghcm_ctx* ctx, const ghcm_config* cfg);
+
+/* ---- plain solve (eikonal_hcm_opencl_ex, src/eikonal_hcm.cpp:186-529) ------
+ * Host buffers: `field` (W*H elements of the given kind) is copied in, the
+ * result is copied out into `time_out` (W*H floats).  Buffers are borrowed
+ * for the duration of the call only.  `stats` may be NULL. */
+int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
+               const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
+               ghcm_stats* stats);
+
+/* Device buffers: `d_field` and `d_time` are device pointers on the context's
+ * device (e.g. from hipMalloc or a torch tensor).  Runs on the context stream
+ * and returns when the solve is complete.  Seeds are host memory. */
+int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind,
+                      uint32_t w, uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
+                      float* d_time, ghcm_stats* stats);
+
+/* ---- batched solve: many uneven instances in one device wave loop ---------- */
+typedef struct ghcm_instance {
+    const void*     d_field;        /* device pointer, field_kind of the batch        */
+    float*          d_time;         /* device pointer, W*H floats                     */
+    uint32_t        w, h;
+    const uint32_t* seeds_xy;       /* host pointer                                   */
+    uint32_t        n_seeds;
+    uint32_t        reserved;
+} ghcm_instance;
+
+int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst,
+                     uint32_t n_inst, ghcm_stats* stats);
+
+/* ---- synthetic fields (seeded, documented generators) ---------------------- */
+/* splitmix64-based; see DESIGN.md for the exact definitions.  Output is FP64
+ * speed (H x W, row-major) so every storage precision derives from one field. */
+int ghcm_gen_ring_speed(uint32_t w, uint32_t h, uint64_t seed, double wall_speed,
+                        double scale, double* speed_out);
+int ghcm_gen_smooth_speed(uint32_t w, uint32_t h, uint64_t seed, double sigma,
+                          double speed_lo, double speed_hi, double* speed_out);
+int ghcm_gen_points(uint32_t w, uint32_t h, uint64_t seed, uint32_t n, uint32_t* xy_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
