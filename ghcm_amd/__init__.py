@@ -82,6 +82,7 @@ EXPORTED_SYMBOLS = (
     "ghcm_abi_version", "ghcm_status_string", "ghcm_last_error", "ghcm_config_default",
     "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
     "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
+    "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download",
     "ghcm_solve_batch", "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
     "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
 )
@@ -120,6 +121,10 @@ def lib() -> ctypes.CDLL:
     L.ghcm_solve.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_device.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_batch.argtypes = [vp, i32, P(Instance), u32, P(Stats)]
+    L.ghcm_dev_alloc.argtypes = [vp, ctypes.c_size_t, P(vp)]
+    L.ghcm_dev_free.argtypes = [vp, vp]
+    L.ghcm_dev_upload.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    L.ghcm_dev_download.argtypes = [vp, vp, vp, ctypes.c_size_t]
     L.ghcm_gen_ring_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,
                                       ctypes.c_double, vp]
     L.ghcm_gen_smooth_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,
@@ -257,6 +262,16 @@ class Context:
             _check(rc)
         return out, st.to_dict()
 
+    def alloc(self, shape, dtype) -> "DeviceArray":
+        """Device buffer on this context's device (freed with the object)."""
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
     def solve_device(self, d_field: int, kind: int, w: int, h: int, seeds, d_time: int) -> dict:
         """Solve a field already resident in device memory (raw device pointers)."""
         s = _seeds_array(seeds)
@@ -278,6 +293,43 @@ class Context:
         st = Stats()
         _check(self._L.ghcm_solve_batch(self._h, kind, arr, n, ctypes.byref(st)))
         return st.to_dict()
+
+
+class DeviceArray:
+    """A device allocation made through the C-ABI (no other GPU runtime needed)."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(ctx._L.ghcm_dev_alloc(ctx._h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(self.ctx._L.ghcm_dev_upload(self.ctx._h, ctypes.c_void_p(self.ptr),
+                                           a.ctypes.data, self.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx._L.ghcm_dev_download(self.ctx._h, out.ctypes.data,
+                                             ctypes.c_void_p(self.ptr), self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and getattr(self.ctx, "_h", None):
+            self.ctx._L.ghcm_dev_free(self.ctx._h, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 # ---- CPU solvers of the C++ API (eikonal_fmm_sequential / eikonal_fim_sequential) ----

@@ -109,7 +109,7 @@ struct ghcm_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
     // device scratch
-    DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status;
+    DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
     PinnedBuf h_ctrl, h_init, h_seeds, h_inst, h_jobs, h_status;
 };
@@ -248,6 +248,7 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->inst, &ctx->tinst, &ctx->qwave, &ctx->key, &ctx->mask, &ctx->tmin,
                       &ctx->list, &ctx->ctrl, &ctx->seeds, &ctx->jobs, &ctx->status,
+                      &ctx->tstat, &ctx->totals,
                       &ctx->field_io, &ctx->time_io})
         b->release();
     for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_init, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
@@ -345,13 +346,16 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     if ((rc = ctx->key.reserve(8ull * T))) return rc;
     if ((rc = ctx->mask.reserve(8ull * T))) return rc;
     if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
-    if ((rc = ctx->list.reserve(8ull * T))) return rc;
+    const uint32_t shard_cap = (T + kShards - 1) / kShards;
+    if ((rc = ctx->list.reserve(8ull * kShards * shard_cap))) return rc;
+    if ((rc = ctx->tstat.reserve(12ull * T))) return rc;
+    if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
     if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
     if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
     if ((rc = ctx->seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_init.reserve(sizeof(GhcmCtrl)))) return rc;
-    if ((rc = ctx->h_ctrl.reserve(2 * sizeof(GhcmCtrl)))) return rc;
+    if ((rc = ctx->h_ctrl.reserve(2 * sizeof(GhcmCtrl) + 64))) return rc;
 
     uint32_t* hs = ctx->h_seeds.as<uint32_t>();
     {
@@ -379,6 +383,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     a.tmin = ctx->tmin.as<float>();
     a.list = ctx->list.as<uint32_t>();
     a.tiles = T;
+    a.shard_cap = shard_cap;
+    a.tstat = ctx->tstat.as<uint32_t>();
     a.ctrl = ctx->ctrl.as<GhcmCtrl>();
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
@@ -393,6 +399,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     HIP_TRY(launch_fill_u32(a.key, 2ull * T, kInfBits, st));
     HIP_TRY(launch_fill_u32(a.mask, 2ull * T, 0u, st));
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a.tmin), T, kInfBits, st));
+    HIP_TRY(launch_fill_u32(a.tstat, 3ull * T, 0u, st));
+    HIP_TRY(launch_fill_u32(ctx->totals.as<uint32_t>(), 6, 0u, st));
     if (n_inst > 1) HIP_TRY(launch_tile_inst(ctx->tinst.as<uint32_t>(), a.inst, n_inst, st));
     HIP_TRY(launch_seed(a, ctx->seeds.as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
 
@@ -427,14 +435,11 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             if (enqueued >= max_waves) break;
             if (chunk < chunk_max) chunk *= 2;
         }
-        HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipEventRecord(ctx->ev_end, st));
+        HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         converged = hc[0].done != 0;
         waves_done = hc[0].busy_waves;
-        visits = hc[0].visits;
-        inner = hc[0].inner_iters;
-        improved = hc[0].improved;
     } else {
         // ---- host priority queue (src/eikonal_hcm.cpp:343-482) ----
         if ((rc = ctx->jobs.reserve(sizeof(HostJob) * n_sel))) return rc;
@@ -506,13 +511,10 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
             ++waves_done;
-            visits += nj;
             for (uint32_t j = 0; j < nj; ++j) {
                 const uint32_t id = hj[j].id;
                 const HostStatus& s = hst[j];
                 smallest[id] = s.smallest_time;
-                inner += s.solves;
-                improved += s.cells_updated;
                 for (int side = 0; side < 4; ++side) {
                     uint32_t nb;
                     if (!s.border[side] || !nbr(id, side, &nb)) continue;
@@ -530,6 +532,14 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         HIP_TRY(hipStreamSynchronize(st));
     }
 
+    // Per-tile counters summed once, outside the timed span.
+    HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
+    unsigned long long* ht = reinterpret_cast<unsigned long long*>(ctx->h_ctrl.as<char>() + 2 * sizeof(GhcmCtrl));
+    HIP_TRY(hipMemcpyAsync(ht, ctx->totals.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    visits = ht[0];
+    inner = ht[1];
+    improved = ht[2];
     float dev_ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&dev_ms, ctx->ev_begin, ctx->ev_end));
     const auto t_wall1 = std::chrono::steady_clock::now();
@@ -590,6 +600,43 @@ extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint
         stats->duration_total_us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+extern "C" int ghcm_dev_alloc(ghcm_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(GHCM_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_dev_free(ghcm_ctx* ctx, void* p) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    if (!p) return GHCM_OK;
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(p));
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_dev_upload(ghcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_dev_download(ghcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GHCM_OK;
 }
 
 extern "C" int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst,

@@ -11,8 +11,13 @@
 //     mask[2][g]     u32  border bits (1 << side) + OPEN_ALL; double-buffered
 //     tmin[g]        f32  smallest own-cell time after the last visit
 //     tinst[g]       u32  instance index (batches only)
-//   list[2][tiles]   u32  wave lists; list[cur] is read by wave w, list[cur^1]
-//                         collects activations for wave w+1
+//     tstat[3][g]    u32  visits / in-tile passes / improved cells of tile g
+//                         (owner-only increments; summed once per solve)
+//   list[2][kShards][shard_cap]  u32  wave lists, sharded by g % kShards so list
+//                         appends spread over kShards counters (each on its own
+//                         128-byte line) instead of contending on one address.
+//                         list[cur] is read by wave w; list[cur^1] collects the
+//                         activations for wave w+1.
 //   ctrl             GhcmCtrl, one per solve
 #pragma once
 
@@ -34,22 +39,24 @@ struct GhcmInst {
     uint32_t    pad;
 };
 
+constexpr uint32_t kShards = 32;        // list shards
+constexpr uint32_t kCounterStride = 32; // u32 between shard counters (128 B)
+
 struct GhcmCtrl {
     uint32_t wave;          // index of the wave being (or about to be) processed
     uint32_t cur;           // list[cur] holds the tiles queued for `wave`
-    uint32_t count[2];      // list lengths
     uint32_t done;          // 1 once a wave starts with an empty list
+    uint32_t n_select;      // n
     uint32_t thr_lo;        // keys < thr_lo are selected
     uint32_t thr_hi;        // keys in [thr_lo, thr_hi) are selected up to `quota`
     uint32_t quota;
     uint32_t quota_used;
-    uint32_t n_select;      // n
     uint32_t busy_waves;    // waves that had work
-    uint32_t pad0;
-    unsigned long long visits;
-    unsigned long long inner_iters;
-    unsigned long long improved;
-    unsigned long long carried;
+    uint32_t total;         // entries in list[cur]
+    uint32_t pad0[6];
+    uint32_t prefix[kShards + 1];   // exclusive prefix of the shard counts of list[cur]
+    uint32_t pad1[31];
+    uint32_t count[2][kShards * kCounterStride];
 };
 
 // Host-scheduled wave: job and status records (src/eikonal_hcm.cpp:22-32,
@@ -76,8 +83,10 @@ struct SolveArgs {
     uint32_t*       key;     // [2][tiles]
     uint32_t*       mask;    // [2][tiles]
     float*          tmin;
-    uint32_t*       list;    // [2][tiles]
+    uint32_t*       list;    // [2][kShards][shard_cap]
     uint32_t        tiles;
+    uint32_t        shard_cap;
+    uint32_t*       tstat;   // [3][tiles]
     GhcmCtrl*       ctrl;
     float           eps;
     uint32_t        iter_cap;

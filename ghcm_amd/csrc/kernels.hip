@@ -2,9 +2,11 @@
 //
 // One wave64 work-group solves one tile of TW x TH cells.  The tile and a one-cell
 // halo ring (corners included) are staged from HBM into LDS, relaxed there with the
-// Godunov upwind update, and the improved cells are committed back with atomicMin on
-// the FP32 bit pattern (all times are >= 0, so unsigned order equals float order and
-// no improvement written by a neighbouring tile can be lost).
+// Godunov upwind update, and the improved cells are written back with plain stores:
+// a tile is solved by at most one work-group per wave and no kernel writes another
+// tile's cells, so each cell has a single writer per launch.  Neighbours that read
+// this tile's edge concurrently see the old or the new value; both are upper bounds,
+// and a neighbour that must see the new value is re-queued for the next wave.
 //
 // Two in-tile styles, as in the reference:
 //   MODE_SYSTOLIC  every cell is updated each pass until no cell improves by >= eps,
@@ -20,8 +22,9 @@
 // Scheduling (device mode): a wave's tiles sit in list[cur]; k_wave_end (one
 // work-group) picks the n lowest keys with a two-level radix histogram and k_solve
 // solves them, carrying the rest to the next list.  Activations append neighbours to
-// list[cur^1] once per wave (atomicMax on qwave).  Host mode takes an explicit job list
-// and writes one status record per job (src/eikonal_hcm.cpp:396-420).
+// list[cur^1] once per wave (atomicMax on qwave), sharded over kShards counters.
+// Host mode takes an explicit job list and writes one status record per job
+// (src/eikonal_hcm.cpp:396-420).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -82,8 +85,15 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Append tile nb to the next wave's list unless it is already there, record why
-// (border side bit) and the smallest candidate time that activated it.
+// Append tile g to list `which` unless it is already there (qwave), in shard g % kShards.
+__device__ __forceinline__ void list_append(const SolveArgs& a, uint32_t which, uint32_t g) {
+    const uint32_t sh = g % kShards;
+    uint32_t slot = atomicAdd(&a.ctrl->count[which][sh * kCounterStride], 1u);
+    a.list[((size_t)which * kShards + sh) * a.shard_cap + slot] = g;
+}
+
+// Queue tile nb for the next wave unless it is already queued, recording why (border
+// side bit) and the smallest candidate time that activated it.
 __device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32_t wave,
                                          uint32_t cur, uint32_t p, uint32_t mask_bits,
                                          float key) {
@@ -91,10 +101,18 @@ __device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32
     atomicMin(&a.key[(size_t)np * a.tiles + nb], __float_as_uint(key));
     atomicOr(&a.mask[(size_t)np * a.tiles + nb], mask_bits);
     uint32_t old = atomicMax(&a.qwave[nb], wave + 2u);
-    if (old < wave + 2u) {
-        uint32_t slot = atomicAdd(&a.ctrl->count[cur ^ 1u], 1u);
-        a.list[(size_t)(cur ^ 1u) * a.tiles + slot] = nb;
+    if (old < wave + 2u) list_append(a, cur ^ 1u, nb);
+}
+
+// Entry i of the current wave list: shard found by a search over the prefix counts.
+__device__ __forceinline__ uint32_t list_entry(const SolveArgs& a, const uint32_t* prefix,
+                                               uint32_t cur, uint32_t i) {
+    uint32_t lo = 0, hi = kShards;           // prefix[lo] <= i < prefix[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= i) lo = mid; else hi = mid;
     }
+    return a.list[((size_t)cur * kShards + lo) * a.shard_cap + (i - prefix[lo])];
 }
 
 template <int TW, int TH, int KIND, int MODE, bool HOST>
@@ -189,8 +207,11 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             if (any && passes >= cap) unfinished = true;
             go = any && passes < cap;
         }
-        // check_border (systolic.cl:8-41): solve the neighbour's adjacent row/column,
-        // store improvements, flag the side if one improved by >= eps.
+        // check_border (systolic.cl:8-41): solve the neighbour's adjacent row/column and
+        // flag the side if a cell would improve by >= eps.  The improved values are not
+        // stored into the neighbour's cells: the flagged neighbour recomputes them from
+        // this tile's committed edge on its next visit, so each cell has one writer per
+        // wave and commits can be plain stores.
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             if (!has_nb[s]) continue;
@@ -204,13 +225,9 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
                 int q = by * PW + bx;
                 float t0 = t_s[q];
                 float tn = halo_solve(s, q);
-                if (tn < t0) {
-                    size_t idx = (size_t)(oy + by - 1) * (size_t)W + (size_t)(ox + bx - 1);
-                    atomicMin(reinterpret_cast<unsigned int*>(in.time) + idx, __float_as_uint(tn));
-                    if (t0 - tn >= eps) {
-                        flags |= 1u << s;
-                        kc[s] = fminf(kc[s], tn);
-                    }
+                if (tn < t0 && t0 - tn >= eps) {
+                    flags |= 1u << s;
+                    kc[s] = fminf(kc[s], tn);
                 }
             }
         }
@@ -298,7 +315,8 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
     }
 
     __syncthreads();
-    // ---- commit improved cells, tile min ----
+    // ---- commit improved cells (plain stores: this work-group is the only writer of
+    // the tile's cells in this wave), tile min ----
     float tmin = __builtin_inff();
     uint32_t improved = 0;
 #pragma unroll
@@ -310,8 +328,7 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
                 float tn = t_s[(y + 1) * PW + x + 1];
                 tmin = fminf(tmin, tn);
                 if (tn < told[k]) {
-                    size_t idx = (size_t)(oy + y) * (size_t)W + (size_t)(ox + x);
-                    atomicMin(reinterpret_cast<unsigned int*>(in.time) + idx, __float_as_uint(tn));
+                    in.time[(size_t)(oy + y) * (size_t)W + (size_t)(ox + x)] = tn;
                     ++improved;
                 }
             }
@@ -339,13 +356,16 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             st.requeue = unfinished ? 1u : 0u;
             a.status[job_slot] = st;
             a.tmin[g] = tmin;
+            a.tstat[g] += 1u;
+            a.tstat[(size_t)a.tiles + g] += passes;
+            a.tstat[2 * (size_t)a.tiles + g] += improved;
         }
     } else {
         if (lane == 0) {
             a.tmin[g] = tmin;
-            atomicAdd(&a.ctrl->visits, 1ull);
-            atomicAdd(&a.ctrl->inner_iters, (unsigned long long)passes);
-            if (improved) atomicAdd(&a.ctrl->improved, (unsigned long long)improved);
+            a.tstat[g] += 1u;
+            a.tstat[(size_t)a.tiles + g] += passes;
+            a.tstat[2 * (size_t)a.tiles + g] += improved;
             if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tmin);
         }
         if (lane < 4 && ((side_flags >> lane) & 1u)) {
@@ -375,15 +395,19 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         process_tile<TW, TH, KIND, MODE, true>(a, job.id, job.border, 0u, 0u, 0u, j,
                                                t_s, c_s, open_s, add_s);
     } else {
+        __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
         if (ctrl->done) return;
+        const uint32_t c = ctrl->total;
+        if (blockIdx.x >= c) return;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
-        const uint32_t c = ctrl->count[cur];
         const uint32_t thr_lo = ctrl->thr_lo, thr_hi = ctrl->thr_hi, quota = ctrl->quota;
         const uint32_t p = wave & 1u;
+        for (int i = lane; i <= (int)kShards; i += 64) s_prefix[i] = ctrl->prefix[i];
+        __syncthreads();
         for (uint32_t i = blockIdx.x; i < c; i += gridDim.x) {
-            const uint32_t g = a.list[(size_t)cur * a.tiles + i];
+            const uint32_t g = list_entry(a, s_prefix, cur, i);
             uint32_t sel = 0, mb = 0;
             if (lane == 0) {
                 uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
@@ -394,10 +418,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
                 mb = *mp;
                 *kp = kInfBits;
                 *mp = 0u;
-                if (!sel) {   // not among the n lowest: carry to the next wave
-                    atomicAdd(&a.ctrl->carried, 1ull);
-                    activate(a, g, wave, cur, p, mb, __uint_as_float(k));
-                }
+                if (!sel) activate(a, g, wave, cur, p, mb, __uint_as_float(k));  // carry
             }
             sel = __shfl(sel, 0);
             mb = __shfl(mb, 0);
@@ -410,41 +431,47 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     }
 }
 
-// Closes the previous wave (swap lists) and selects the n lowest keys of the next one.
-// One work-group of 1024 threads; the histogram runs only when the list exceeds n.
+// Closes the previous wave (swap lists), computes the shard prefix of the new list and,
+// when it holds more than n tiles, a threshold selecting its n lowest keys: a two-level
+// radix histogram over key bits [30:19] then [18:7] (keys are non-negative floats, so
+// their bit patterns order like the values).  One work-group of 1024 threads.
 __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) {
     __shared__ uint32_t hist[4096];
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t s_prefix[kShards + 1];
     __shared__ uint32_t s_bin, s_below;
     GhcmCtrl* ctrl = a.ctrl;
     const int tid = threadIdx.x;
     if (ctrl->done) return;
-    __syncthreads();
-    if (tid == 0 && !first) {
-        ctrl->count[ctrl->cur] = 0u;
-        ctrl->cur ^= 1u;
-        ctrl->wave += 1u;
+    uint32_t cur = ctrl->cur;
+    const uint32_t wave0 = ctrl->wave;
+    if (!first) {
+        // list[cur] was consumed by the wave that just ran: empty it and swap.
+        if (tid < (int)kShards) ctrl->count[cur][tid * kCounterStride] = 0u;
+        cur ^= 1u;
     }
-    __syncthreads();
-    const uint32_t cur = ctrl->cur;
-    const uint32_t c = ctrl->count[cur];
-    const uint32_t n = ctrl->n_select;
-    const uint32_t p = ctrl->wave & 1u;
-    if (c == 0u) {
-        if (tid == 0) ctrl->done = 1u;
-        return;
-    }
-    if (c <= n) {
-        if (tid == 0) {
-            ctrl->thr_lo = 0xFFFFFFFFu;
-            ctrl->thr_hi = 0xFFFFFFFFu;
-            ctrl->quota = 0u;
-            ctrl->quota_used = 0u;
-            ctrl->busy_waves += 1u;
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t sh = 0; sh < kShards; ++sh) {
+            s_prefix[sh] = acc;
+            acc += ctrl->count[cur][sh * kCounterStride];
         }
-        return;
+        s_prefix[kShards] = acc;
     }
-    // Level 1: bins of key bits [30:19] (exponent + 4 mantissa bits).
-    const uint32_t* lst = a.list + (size_t)cur * a.tiles;
+    __syncthreads();
+    const uint32_t c = s_prefix[kShards];
+    const uint32_t n = ctrl->n_select;
+    if (tid <= (int)kShards) ctrl->prefix[tid] = s_prefix[tid];
+    if (tid == 0) {
+        if (!first) { ctrl->cur = cur; ctrl->wave = wave0 + 1u; }
+        ctrl->total = c;
+        ctrl->quota_used = 0u;
+        if (c == 0u) ctrl->done = 1u;
+        else ctrl->busy_waves += 1u;
+        if (c <= n) { ctrl->thr_lo = 0xFFFFFFFFu; ctrl->thr_hi = 0xFFFFFFFFu; ctrl->quota = 0u; }
+    }
+    if (c == 0u || c <= n) return;
+    const uint32_t p = (wave0 + (first ? 0u : 1u)) & 1u;
     const uint32_t* key = a.key + (size_t)p * a.tiles;
     uint32_t lo_base = 0u, below = 0u;
     for (int level = 0; level < 2; ++level) {
@@ -452,18 +479,31 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
         for (int i = tid; i < 4096; i += 1024) hist[i] = 0u;
         __syncthreads();
         for (uint32_t i = tid; i < c; i += 1024) {
-            uint32_t k = key[lst[i]];
+            uint32_t k = key[list_entry(a, s_prefix, cur, i)];
             if (level == 1 && (k >> 19) != (lo_base >> 19)) continue;
             atomicAdd(&hist[(k >> shift) & 0xFFFu], 1u);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t cum = below, b = 0;
-            for (; b < 4096; ++b) {
-                if (cum + hist[b] >= n) break;
-                cum += hist[b];
+        // Each thread owns 4 consecutive bins; exclusive scan of the 1024 partial sums.
+        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2],
+                       h3 = hist[4 * tid + 3];
+        part[tid] = h0 + h1 + h2 + h3;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t v = tid >= off ? part[tid - off] : 0u;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        const uint32_t incl = part[tid] + below;
+        const uint32_t excl = incl - (h0 + h1 + h2 + h3);
+        if (excl < n && incl >= n) {           // the crossing bin is one of this thread's 4
+            uint32_t cum = excl, b = 4 * tid;
+            const uint32_t hh[4] = {h0, h1, h2, h3};
+            for (int j = 0; j < 4; ++j) {
+                if (cum + hh[j] >= n) { b = 4 * tid + j; break; }
+                cum += hh[j];
             }
-            if (b == 4096) b = 4095;
             s_bin = b;
             s_below = cum;
         }
@@ -473,12 +513,31 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
         __syncthreads();
     }
     if (tid == 0) {
-        ctrl->thr_lo = lo_base;                 // keys < lo_base: below `below`
+        ctrl->thr_lo = lo_base;                 // `below` keys are < lo_base
         ctrl->thr_hi = lo_base + (1u << 7);
         ctrl->quota = n - below;
-        ctrl->quota_used = 0u;
-        ctrl->busy_waves += 1u;
     }
+}
+
+// Sums the per-tile counters into out[0..2] (visits, passes, improved cells).
+__global__ void k_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out) {
+    __shared__ unsigned long long red[3][256];
+    unsigned long long v[3] = {0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < tiles;
+         i += (size_t)gridDim.x * blockDim.x) {
+        v[0] += tstat[i];
+        v[1] += tstat[(size_t)tiles + i];
+        v[2] += tstat[2 * (size_t)tiles + i];
+    }
+    for (int j = 0; j < 3; ++j) red[j][threadIdx.x] = v[j];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 3; ++j) atomicAdd(&out[j], red[j][0]);
 }
 
 // ---- initialisation kernels (fill kernels rather than memsets) ----
@@ -514,10 +573,7 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
         atomicMin(&a.key[t], 0u);
         atomicOr(&a.mask[t], kMaskOpenAll);
         uint32_t old = atomicMax(&a.qwave[t], 1u);
-        if (old < 1u) {
-            uint32_t slot = atomicAdd(&a.ctrl->count[0], 1u);
-            a.list[slot] = t;
-        }
+        if (old < 1u) list_append(a, 0u, t);
     }
 }
 
@@ -572,6 +628,15 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s) {
     unsigned blocks = n_inst < 1024 ? n_inst : 1024;
     hipLaunchKernelGGL(k_tile_inst, dim3(blocks), dim3(256), 0, s, tinst, inst, n_inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out,
+                              hipStream_t s) {
+    unsigned blocks = (tiles + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_reduce_stats, dim3(blocks), dim3(256), 0, s, tstat, tiles, out);
     return hipGetLastError();
 }
 

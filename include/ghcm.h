@@ -147,6 +147,13 @@ int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind,
                       uint32_t w, uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
                       float* d_time, ghcm_stats* stats);
 
+/* ---- device memory on the context's device (plumbing for ghcm_solve_device and
+ * ghcm_solve_batch, so hosts need no other GPU runtime in-process) ---------------- */
+int ghcm_dev_alloc(ghcm_ctx* ctx, size_t bytes, void** d_out);
+int ghcm_dev_free(ghcm_ctx* ctx, void* d_ptr);
+int ghcm_dev_upload(ghcm_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int ghcm_dev_download(ghcm_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+
 /* ---- batched solve: many uneven instances in one device wave loop ---------- */
 typedef struct ghcm_instance {
     const void*     d_field;        /* device pointer, field_kind of the batch        */

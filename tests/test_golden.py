@@ -1,0 +1,54 @@
+"""Committed fixtures (tests/golden/) against the oracle and the C++ CPU solvers -- CPU."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import ghcm_amd as g
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _arr(v):
+    return np.array([[np.inf if x == "inf" else x for x in row] for row in v], dtype=np.float64)
+
+
+KATS = json.load(open(os.path.join(GOLDEN, "hand_kats.json")))
+
+
+@pytest.mark.parametrize("name", sorted(KATS))
+def test_hand_kats(name):
+    k = KATS[name]
+    cost = _arr(k["cost"])
+    seeds = k["seeds"]
+    solvers = {
+        "fmm64": lambda: oracle.fmm_f64(cost, seeds),
+        "fmm32": lambda: oracle.fmm_f32(cost.astype(np.float32), seeds),
+        "fim32": lambda: oracle.fim_f32(cost.astype(np.float32), seeds, 1e-6)[0],
+        "cpp_fmm": lambda: g.fmm_sequential(cost.astype(np.float32), seeds)[0],
+    }
+    for sname, f in solvers.items():
+        t = f()
+        if "time" in k:
+            exp = _arr(k["time"])
+            fin = np.isfinite(exp)
+            assert np.array_equal(fin, np.isfinite(t)), sname
+            assert np.allclose(t[fin], exp[fin], rtol=1e-6), sname
+        for key, val in k.get("partial", {}).items():
+            y, x = (int(v) for v in key.split(","))
+            assert t[y, x] == pytest.approx(val, rel=1e-6), (sname, key)
+
+
+def test_c1_fixture_regression():
+    d = np.load(os.path.join(GOLDEN, "c1_ring256_fmm64.npz"))
+    speed = g.gen_ring_speed(256, 256, 1, 1e-3, 1.0)
+    assert np.array_equal(speed.astype(np.float32), d["speed"])
+    seeds = d["seeds"]
+    t = oracle.fmm_f64(1.0 / speed, seeds)
+    assert np.allclose(t, d["time"], rtol=1e-6)
+    t32, _ = g.fmm_sequential(g.cost_from_speed(speed), seeds)
+    m = oracle.error_metrics(t32, d["time"].astype(np.float64))
+    assert m["finiteness_mismatch"] == 0
+    assert m["linf"] <= 1e-5 * m["max_ref"]

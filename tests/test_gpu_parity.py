@@ -1,0 +1,179 @@
+"""Parity of the HIP path (libghcm.so) with the oracle -- needs an MI355X.
+
+Bar: max |dT| <= 1e-4 * max T against the oracle's FP32 FMM (restating
+src/eikonal_fmm.cpp), identical finite/INF pattern, T(seed) = 0, and the discrete
+Godunov residual small at every cell.  Every call goes through the C-ABI.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import ghcm_amd as g
+import oracle
+from conftest import fixed_point_gap, ulp32
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TOL = 1e-4          # max |dT| <= TOL * max T (north-star tolerance for the FP32 plain form)
+
+TILES = [(8, 8), (16, 16), (32, 8), (32, 32)]
+KERNELS = [g.KERNEL_SYSTOLIC, g.KERNEL_FIM]
+
+
+def check(t, ref, seeds, cost=None, tol=TOL, label="", eps=1e-5):
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(t)), f"{label}: INF pattern differs"
+    if fin.any():
+        err = float(np.max(np.abs(t[fin].astype(np.float64) - ref[fin])))
+        assert err <= tol * float(ref[fin].max()) + 1e-6, f"{label}: max|dT|={err}"
+    for x, y in np.asarray(seeds).reshape(-1, 2):
+        assert t[y, x] == 0.0, f"{label}: seed ({x},{y}) = {t[y, x]}"
+    assert np.all(t[fin] >= 0)
+    if cost is not None:
+        # converged to the discrete fixed point within a few ulps plus epsilon
+        gap = fixed_point_gap(t, cost, seeds)
+        assert np.all(gap <= 4 * ulp32(t) + 2 * eps), f"{label}: fixed-point gap {gap.max()}"
+
+
+@pytest.fixture(scope="module")
+def c1():
+    d = np.load(os.path.join(GOLDEN, "c1_ring256_fmm64.npz"))
+    speed = g.gen_ring_speed(256, 256, 1, 1e-3, 1.0)
+    cost = g.cost_from_speed(speed)
+    return cost, d["seeds"], oracle.fmm_f32(cost, d["seeds"]), d["time"]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("tile", TILES)
+@pytest.mark.parametrize("sched", [g.SCHED_DEVICE_TOPN, g.SCHED_HOST_PQ])
+def test_c1_rings_every_config(c1, kernel, tile, sched):
+    cost, seeds, ref32, ref64 = c1
+    print(f"case c1 kernel={kernel} tile={tile} sched={sched}", flush=True)
+    with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, scheduler=sched,
+                   compute_unit_overload=4.0) as ctx:
+        t, st = ctx.solve(cost, seeds)
+    assert st["converged"] == 1 and st["waves"] > 0
+    check(t, ref32, seeds, cost, label="vs fmm32")
+    check(t, ref64.astype(np.float64), seeds, label="vs fmm64 fixture")
+
+
+@pytest.mark.parametrize("n_active", [1, 3, 64, 100000])
+def test_top_n_sizes(c1, n_active):
+    """n smaller than the active set exercises the device top-n selection."""
+    cost, seeds, ref32, _ = c1
+    with g.Context(0, n_active=n_active) as ctx:
+        t, st = ctx.solve(cost, seeds)
+    check(t, ref32, seeds, label=f"n={n_active}")
+
+
+SHAPES = [(1, 1), (1, 37), (37, 1), (13, 29), (100, 70), (3, 250), (257, 129)]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("shape", SHAPES)
+def test_ragged_obstacles_multi_seed(rng, kernel, shape):
+    h, w = shape
+    cost = rng.uniform(0.1, 10.0, (h, w)).astype(np.float32)
+    cost[rng.random((h, w)) < 0.12] = np.inf
+    seeds = {(0, 0), (w - 1, h - 1), (int(rng.integers(0, w)), int(rng.integers(0, h)))}
+    seeds = sorted(seeds)
+    for x, y in seeds:
+        cost[y, x] = 1.0
+    ref = oracle.fmm_f32(cost, seeds)
+    for tile in ((8, 8), (32, 8)):
+        print(f"case ragged {shape} kernel={kernel} tile={tile}", flush=True)
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=7) as ctx:
+            t, _ = ctx.solve(cost, seeds)
+        check(t, ref, seeds, cost, label=f"{shape} {tile}")
+
+
+def test_adjacent_seeds_and_edges():
+    cost = np.full((40, 50), 2.0, dtype=np.float32)
+    seeds = [(7, 7), (8, 7), (7, 8), (0, 39), (49, 0), (16, 16)]      # adjacent + tile corners
+    ref = oracle.fmm_f32(cost, seeds)
+    with g.Context(0) as ctx:
+        t, _ = ctx.solve(cost, seeds)
+    check(t, ref, seeds, cost)
+
+
+def test_unreachable_region_stays_inf():
+    cost = np.ones((64, 64), dtype=np.float32)
+    cost[:, 20] = np.inf
+    cost[30, 40:] = np.inf
+    seeds = [(3, 3)]
+    with g.Context(0, tile_w=16, tile_h=16) as ctx:
+        t, _ = ctx.solve(cost, seeds)
+    assert np.all(np.isinf(t[:, 20:]))
+    check(t, oracle.fmm_f32(cost, seeds), seeds)
+
+
+def test_no_seeds_all_inf():
+    cost = np.ones((20, 20), dtype=np.float32)
+    with g.Context(0) as ctx:
+        t, st = ctx.solve(cost, np.zeros((0, 2), np.uint32))
+    assert np.all(np.isinf(t)) and st["converged"] == 1
+
+
+def test_strip_prefix_sums():
+    cost = np.random.default_rng(1).uniform(0.1, 3.0, (1, 1000)).astype(np.float32)
+    with g.Context(0, tile_w=32, tile_h=8) as ctx:
+        t, _ = ctx.solve(cost, [(0, 0)])
+    exp = np.concatenate([[0.0], np.cumsum(cost[0, 1:].astype(np.float64))])
+    assert np.allclose(t[0], exp, rtol=1e-5)
+
+
+def test_epsilon_zero_exact_convergence(c1):
+    cost, seeds, ref32, _ = c1
+    for kernel in KERNELS:
+        with g.Context(0, min_change=0.0, kernel=kernel) as ctx:
+            t, _ = ctx.solve(cost, seeds)
+        check(t, ref32, seeds, cost, tol=1e-5, label=f"eps=0 kernel={kernel}")
+
+
+def test_metamorphic_configs_agree(rng):
+    speed = g.gen_smooth_speed(300, 220, seed=5, sigma=8.0)
+    cost = g.cost_from_speed(speed)
+    seeds = g.gen_points(300, 220, 6, 3)
+    outs = []
+    for tile in TILES:
+        for kernel in KERNELS:
+            with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=50) as ctx:
+                outs.append(ctx.solve(cost, seeds)[0])
+    ref = oracle.fmm_f32(cost, seeds)
+    for t in outs:
+        check(t, ref, seeds, cost)
+        assert np.max(np.abs(t - outs[0])) <= TOL * ref.max()
+
+
+def test_cxx_api_path(c1):
+    cost, seeds, ref32, _ = c1
+    for kernel in KERNELS:
+        t, st = g.hcm_solve_cxx(cost, seeds, block_size=16, compute_unit_overload=2.0,
+                                kernel=kernel)
+        check(t, ref32, seeds, label=f"c++ api kernel={kernel}")
+        assert st["tile_w"] == 16
+
+
+def test_errors_reported():
+    cost = np.ones((16, 16), dtype=np.float32)
+    with g.Context(0) as ctx:
+        with pytest.raises(g.GhcmError) as e:
+            ctx.solve(cost, [(16, 0)])
+        assert e.value.status == g.ERR_INVALID_ARG
+        ctx.set_config(max_waves=1, waves_per_poll=1)
+        big = np.ones((256, 256), dtype=np.float32)
+        with pytest.raises(g.GhcmError) as e:
+            ctx.solve(big, [(0, 0)])
+        assert e.value.status == g.ERR_NOT_CONVERGED
+        ctx.set_config(max_waves=0, waves_per_poll=0)
+        t, _ = ctx.solve(cost, [(0, 0)])           # context still usable
+        assert t[0, 0] == 0.0
+
+
+def test_context_reuse_after_smaller_and_larger():
+    with g.Context(0, tile_w=16, tile_h=16) as ctx:
+        for (h, w) in ((300, 200), (17, 33), (512, 700)):
+            cost = np.ones((h, w), dtype=np.float32)
+            t, _ = ctx.solve(cost, [(w // 2, h // 2)])
+            check(t, oracle.fmm_f32(cost, [(w // 2, h // 2)]), [(w // 2, h // 2)])

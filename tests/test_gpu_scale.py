@@ -1,0 +1,99 @@
+"""Full-size (4096^2) and precision/batch checks of the HIP path -- needs an MI355X."""
+import numpy as np
+import pytest
+
+import ghcm_amd as g
+import oracle
+from conftest import fixed_point_gap, ulp32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c2():
+    speed = g.gen_smooth_speed(4096, 4096, 20261202, 32.0, 0.1, 10.0)
+    seeds = g.gen_points(4096, 4096, 20261203, 4)
+    cost = g.cost_from_speed(speed)
+    return speed, cost, seeds
+
+
+def test_c2_full_size_parity(c2):
+    speed, cost, seeds = c2
+    ref = oracle.fmm_f32(cost, seeds)
+    for tile, kernel in (((16, 16), g.KERNEL_SYSTOLIC), ((8, 8), g.KERNEL_FIM)):
+        print(f"case c2 tile={tile} kernel={kernel}", flush=True)
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=16384) as ctx:
+            dc = ctx.to_device(cost)
+            dt = ctx.alloc(cost.shape, np.float32)
+            st = ctx.solve_device(dc.ptr, g.FIELD_COST_F32, 4096, 4096, seeds, dt.ptr)
+            t = dt.download()
+            dc.free()
+            dt.free()
+        assert st["converged"] == 1
+        assert np.all(np.isfinite(t))
+        err = np.max(np.abs(t.astype(np.float64) - ref))
+        assert err <= 1e-4 * ref.max(), err
+        assert np.all(fixed_point_gap(t, cost, seeds) <= 4 * ulp32(t) + 2e-5)
+        for x, y in seeds:
+            assert t[y, x] == 0.0
+
+
+@pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
+def test_precision_storage_vs_fp64(kind, tol):
+    """Speed stored as FP32/FP16/BF16, times and updates in FP32, against the FP64 FMM."""
+    speed = g.gen_smooth_speed(512, 512, 77, 8.0, 0.1, 10.0)
+    seeds = g.gen_points(512, 512, 78, 3)
+    ref64 = oracle.fmm_f64(1.0 / speed, seeds)
+    k = g.KINDS[kind]
+    if kind == "speed_f32":
+        field = speed.astype(np.float32)
+    elif kind == "speed_f16":
+        field = speed.astype(np.float16)
+    else:
+        field = g.to_bf16(speed.astype(np.float32))
+    with g.Context(0, tile_w=16, tile_h=16) as ctx:
+        t, _ = ctx.solve(field, seeds, kind=k)
+    m = oracle.error_metrics(t, ref64)
+    print(f"{kind}: linf={m['linf']:.3e} l1={m['l1']:.3e} max={m['max_ref']:.1f}")
+    assert m["finiteness_mismatch"] == 0
+    assert m["linf"] <= tol * m["max_ref"]
+
+
+def test_ring_contrast_1e4_f16():
+    """1e-4..1 contrast (walls at speed 1e-4): FP16 speed covers the range."""
+    speed = g.gen_ring_speed(512, 512, 5, 1e-4, 2.0)
+    seeds = [(256, 256)]
+    ref64 = oracle.fmm_f64(1.0 / speed, seeds)
+    for kind, field in (("speed_f32", speed.astype(np.float32)), ("speed_f16", speed.astype(np.float16))):
+        with g.Context(0, tile_w=16, tile_h=16) as ctx:
+            t, _ = ctx.solve(field, seeds, kind=g.KINDS[kind])
+        m = oracle.error_metrics(t, ref64)
+        assert m["finiteness_mismatch"] == 0
+        assert m["linf"] <= 1e-3 * m["max_ref"], (kind, m)
+
+
+def test_batch_equals_one_at_a_time():
+    rng = np.random.default_rng(3)
+    insts = []
+    for i in range(12):
+        w, h = (int(v) for v in rng.choice(np.arange(64, 449, 64), 2))
+        if i % 2:
+            speed = g.gen_ring_speed(w, h, 100 + i, 1e-3, min(w, h) / 256)
+        else:
+            speed = g.gen_smooth_speed(w, h, 100 + i, 8.0, 0.1, 10.0)
+        seeds = g.gen_points(w, h, 200 + i, 1 + i % 4)
+        cost = g.cost_from_speed(speed)
+        insts.append((cost, seeds))
+    with g.Context(0, tile_w=16, tile_h=16, n_active=4096) as ctx:
+        dev = [(ctx.to_device(c), ctx.alloc(c.shape, np.float32)) for c, _ in insts]
+        st = ctx.solve_batch(g.FIELD_COST_F32, [(dc.ptr, dt.ptr, c.shape[1], c.shape[0], s)
+                                                for (dc, dt), (c, s) in zip(dev, insts)])
+        assert st["converged"] == 1
+        for (dc, dt), (c, s) in zip(dev, insts):
+            ref = oracle.fmm_f32(c, s)
+            tb = dt.download()
+            single, _ = ctx.solve(c, s)
+            fin = np.isfinite(ref)
+            assert np.array_equal(fin, np.isfinite(tb))
+            assert np.max(np.abs(tb[fin] - ref[fin])) <= 1e-4 * ref[fin].max()
+            assert np.max(np.abs(tb[fin] - single[fin])) <= 1e-4 * ref[fin].max()
