@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the gHCM hot path on one MI355X.
+
+Workload (the 4096^2 plain solve): a 4096 x 4096 FP32 cost field, cost = 1/speed with
+speed = 10^(-1 + 2 (z - min z)/(max z - min z)), z a standard-normal field (seed
+20261202) smoothed by a Gaussian of sigma 32 cells; 4 point sources drawn uniformly
+(seed 20261203).  A "step" is one complete solve: initialise the time field, run the
+device wave loop until the active-tile list drains.  Field and time buffers are resident
+in HBM before the timed region starts.
+
+Prints ONE JSON line on stdout: Mcells/s of the whole solve, the roofline entry of the
+dominant kernel (k_solve: algorithmic bytes per launch / average launch duration, HIP
+events on the solver stream), and the CPU baseline (the oracle's FP32 FMM on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+W = H = 4096
+FIELD_SEED, POINT_SEED = 20261202, 20261203
+METRIC = "Mcells/s at 4096² (% HBM peak); batched solves/s vs plain; p99 latency"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_c2_traffic.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per k_solve launch from the committed rocprofv3 --pmc summary, if any."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(cost, seeds, trials: int):
+    """The oracle's FP32 FMM (restating src/eikonal_fmm.cpp) on one host core."""
+    import oracle
+    old = os.sched_getaffinity(0)
+    try:
+        os.sched_setaffinity(0, {min(old)})
+    except OSError:
+        pass
+    times, ref = [], None
+    try:
+        for _ in range(trials):
+            t0 = time.perf_counter()
+            ref = oracle.fmm_f32(cost, seeds)
+            times.append(time.perf_counter() - t0)
+    finally:
+        try:
+            os.sched_setaffinity(0, old)
+        except OSError:
+            pass
+    med = statistics.median(times)
+    return ref, {"value": round(W * H / med / 1e6, 3), "unit": "Mcells/s", "cores": 1,
+                 "kind": "port",
+                 "sample": f"full {W}x{H} workload, oracle FP32 FMM, {trials} trials, median "
+                           f"{med:.2f} s, pinned to one core"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tile", default="16x16")
+    ap.add_argument("--kernel", default="systolic", choices=["systolic", "fim"])
+    ap.add_argument("--n-active", type=int, default=16384)
+    ap.add_argument("--cpu-trials", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this size first (rocprofv3 --pmc scaling)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the gHCM path is a single-"
+              "device solve; multi-GPU sharding is described in DESIGN.md). Use --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import ghcm_amd as g
+
+    tw, th = (int(v) for v in args.tile.split("x"))
+    kernel = g.KERNEL_FIM if args.kernel == "fim" else g.KERNEL_SYSTOLIC
+    log(f"bench: generating {W}x{H} workload")
+    speed = g.gen_smooth_speed(W, H, FIELD_SEED, 32.0, 0.1, 10.0)
+    seeds = g.gen_points(W, H, POINT_SEED, 4)
+    cost = g.cost_from_speed(speed)
+
+    ctx = g.Context(0, tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
+                    profile_kernels=1)
+    info = g.device_info(0)
+    d_cost = ctx.to_device(cost)
+    d_time = ctx.alloc(cost.shape, np.float32)
+    if args.copy_calibration_mib:
+        n = args.copy_calibration_mib << 20
+        src, dst = ctx.alloc((n,), np.uint8), ctx.alloc((n,), np.uint8)
+        g.lib().ghcm_dev_copy(ctx._h, dst.ptr, src.ptr, n)
+        src.free()
+        dst.free()
+
+    def step():
+        return ctx.solve_device(d_cost.ptr, g.FIELD_COST_F32, W, H, seeds, d_time.ptr)
+
+    for i in range(args.warmup):
+        st = step()
+        log(f"bench: warmup {i + 1}/{args.warmup} {st['duration_device_us'] / 1e3:.2f} ms")
+
+    # Timed region: each solve_device call returns after the solver stream drained.
+    sts = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sts.append(step())
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = W * H / (ms_per_step / 1e3) / 1e6
+    log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.1f} Mcells/s")
+
+    t_gpu = d_time.download()
+    launches = sum(s["solve_launches"] for s in sts)
+    kern_us = sum(s["solve_kernel_us"] for s in sts)
+    visits = sum(s["tile_visits"] for s in sts)
+    bpv = sts[0]["bytes_per_visit"]
+    bytes_per_launch = visits * bpv / max(launches, 1)
+    avg_launch_us = kern_us / max(launches, 1)
+    achieved = bytes_per_launch / (avg_launch_us * 1e-6) / 1e9
+    traffic = read_traffic()
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None if traffic is None else round(traffic, 1),
+            "kernel": "k_solve", "avg_launch_us": round(avg_launch_us, 3),
+            "bytes_per_launch": round(bytes_per_launch, 1),
+            "launches_per_step": launches / args.steps}
+
+    out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcells/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded smooth log-normal speed field)",
+           "config": {"workload": f"plain gHCM {W}x{H} FP32 cost, smooth speed [0.1,10] "
+                                  f"sigma 32, 4 point sources",
+                      "tile": f"{tw}x{th}", "kernel": args.kernel,
+                      "n_active": args.n_active, "scheduler": "device_topn",
+                      "fim_minimum_change": 1e-5, "parallelism": "single_gpu",
+                      "device": info["arch"]},
+           "solve": {"waves": sts[0]["waves"], "tile_visits": sts[0]["tile_visits"],
+                     "work_amplification": round(sts[0]["tile_visits"] * tw * th / (W * H), 3),
+                     "inner_passes_per_visit": round(sts[0]["inner_iterations"] /
+                                                     max(1, sts[0]["tile_visits"]), 2),
+                     "device_ms": round(statistics.median(s["duration_device_us"] for s in sts) / 1e3, 4)},
+           "roofline": roof}
+
+    if not args.no_cpu_baseline:
+        log(f"bench: CPU baseline ({args.cpu_trials} trials of the oracle FMM)")
+        ref, cb = cpu_baseline(cost, seeds, args.cpu_trials)
+        out["cpu_baseline"] = cb
+        err = float(np.max(np.abs(t_gpu.astype(np.float64) - ref)))
+        out["parity"] = {"linf_vs_cpu_fmm": err, "linf_rel": err / float(ref.max()),
+                         "tolerance_rel": 1e-4}
+    else:
+        out["cpu_baseline"] = None
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

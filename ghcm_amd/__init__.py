@@ -45,7 +45,8 @@ class Config(ctypes.Structure):
                 ("n_active", ctypes.c_uint32), ("min_change", ctypes.c_float),
                 ("cache_mode", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("scheduler", ctypes.c_int32), ("inner_iter_cap", ctypes.c_uint32),
-                ("max_waves", ctypes.c_uint32), ("waves_per_poll", ctypes.c_uint32)]
+                ("max_waves", ctypes.c_uint32), ("waves_per_poll", ctypes.c_uint32),
+                ("profile_kernels", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -56,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("inner_iterations", ctypes.c_uint64), ("cells_improved", ctypes.c_uint64),
                 ("converged", ctypes.c_uint32), ("tile_w", ctypes.c_uint32),
                 ("tile_h", ctypes.c_uint32), ("n_active", ctypes.c_uint32),
-                ("bytes_per_visit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("bytes_per_visit", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("solve_kernel_us", ctypes.c_double), ("solve_launches", ctypes.c_uint64)]
 
     def to_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -82,7 +84,7 @@ EXPORTED_SYMBOLS = (
     "ghcm_abi_version", "ghcm_status_string", "ghcm_last_error", "ghcm_config_default",
     "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
     "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
-    "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download",
+    "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download", "ghcm_dev_copy",
     "ghcm_solve_batch", "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
     "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
 )
@@ -125,6 +127,7 @@ def lib() -> ctypes.CDLL:
     L.ghcm_dev_free.argtypes = [vp, vp]
     L.ghcm_dev_upload.argtypes = [vp, vp, vp, ctypes.c_size_t]
     L.ghcm_dev_download.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    L.ghcm_dev_copy.argtypes = [vp, vp, vp, ctypes.c_size_t]
     L.ghcm_gen_ring_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,
                                       ctypes.c_double, vp]
     L.ghcm_gen_smooth_speed.argtypes = [u32, u32, ctypes.c_uint64, ctypes.c_double,

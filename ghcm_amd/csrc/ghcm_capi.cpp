@@ -108,6 +108,7 @@ struct ghcm_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     hipEvent_t ev_poll[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_prof;    // profile_kernels: begin/end pairs, 2 chunks
     // device scratch
     DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
@@ -155,6 +156,7 @@ extern "C" void ghcm_config_default(ghcm_config* c) {
     c->inner_iter_cap = 0;
     c->max_waves = 0;
     c->waves_per_poll = 0;
+    c->profile_kernels = 0;
 }
 
 extern "C" int ghcm_device_count(int* count) {
@@ -257,6 +259,8 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
     if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
     for (hipEvent_t ev : ctx->ev_poll)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : ctx->ev_prof)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -406,7 +410,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
 
     uint64_t max_waves = cfg.max_waves ? cfg.max_waves : (uint64_t)8 * T + 100000;
     bool converged = false;
-    uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0;
+    uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0, kern_launches = 0;
+    double kern_us = 0.0;
 
     if (!host_sched) {
         // ---- device-resident wave loop; the host only polls a done flag ----
@@ -414,14 +419,41 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         GhcmCtrl* hc = ctx->h_ctrl.as<GhcmCtrl>();
         uint32_t chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
         const uint32_t chunk_max = cfg.waves_per_poll ? cfg.waves_per_poll : 64;
+        const bool prof = cfg.profile_kernels != 0;
+        if (prof) {
+            const size_t need = 2ull * 2 * chunk_max;
+            while (ctx->ev_prof.size() < need) {
+                hipEvent_t ev;
+                HIP_TRY(hipEventCreate(&ev));
+                ctx->ev_prof.push_back(ev);
+            }
+        }
+        uint32_t chunk_len[2] = {0, 0};
+        auto harvest = [&](int sl) -> int {        // add the slot's launch times
+            for (uint32_t k = 0; k < chunk_len[sl]; ++k) {
+                float ms = 0.0f;
+                hipEvent_t b = ctx->ev_prof[(size_t)(sl * chunk_max + k) * 2];
+                hipEvent_t e = ctx->ev_prof[(size_t)(sl * chunk_max + k) * 2 + 1];
+                HIP_TRY(hipEventElapsedTime(&ms, b, e));
+                kern_us += 1000.0 * ms;
+                ++kern_launches;
+            }
+            chunk_len[sl] = 0;
+            return GHCM_OK;
+        };
         uint64_t enqueued = 0;
         int inflight = 0, slot = 0;
         // Two chunks in flight: the GPU never waits on the host's poll.
         for (;;) {
             for (uint32_t k = 0; k < chunk; ++k) {
-                HIP_TRY(launch(a, n_sel, st));
+                if (prof)
+                    HIP_TRY(launch(a, n_sel, st, ctx->ev_prof[(size_t)(slot * chunk_max + k) * 2],
+                                   ctx->ev_prof[(size_t)(slot * chunk_max + k) * 2 + 1]));
+                else
+                    HIP_TRY(launch(a, n_sel, st, nullptr, nullptr));
                 HIP_TRY(launch_wave_end(a, 0u, st));
             }
+            chunk_len[slot] = prof ? chunk : 0;
             enqueued += chunk;
             HIP_TRY(hipMemcpyAsync(&hc[slot], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
             HIP_TRY(hipEventRecord(ctx->ev_poll[slot], st));
@@ -429,6 +461,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             slot ^= 1;
             if (inflight == 2) {
                 HIP_TRY(hipEventSynchronize(ctx->ev_poll[slot]));
+                if ((rc = harvest(slot))) return rc;
                 --inflight;
                 if (hc[slot].done) { converged = true; break; }
             }
@@ -438,6 +471,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         HIP_TRY(hipEventRecord(ctx->ev_end, st));
         HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        if ((rc = harvest(0)) || (rc = harvest(1))) return rc;
         converged = hc[0].done != 0;
         waves_done = hc[0].busy_waves;
     } else {
@@ -481,6 +515,11 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             bool operator<(const Ent& o) const { return key > o.key; }
         };
         std::priority_queue<Ent> pq;
+        while (cfg.profile_kernels && ctx->ev_prof.size() < 2) {
+            hipEvent_t ev;
+            HIP_TRY(hipEventCreate(&ev));
+            ctx->ev_prof.push_back(ev);
+        }
         for (uint32_t i = 0; i < n_inst; ++i)
             for (uint32_t s = 0; s < hi[i].n_seeds; ++s) {
                 const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
@@ -507,9 +546,18 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             }
             a.n_jobs = nj;
             HIP_TRY(hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * nj, hipMemcpyHostToDevice, st));
-            HIP_TRY(launch(a, nj, st));
+            if (cfg.profile_kernels)
+                HIP_TRY(launch(a, nj, st, ctx->ev_prof[0], ctx->ev_prof[1]));
+            else
+                HIP_TRY(launch(a, nj, st, nullptr, nullptr));
             HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
+            if (cfg.profile_kernels) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_prof[0], ctx->ev_prof[1]));
+                kern_us += 1000.0 * ms;
+                ++kern_launches;
+            }
             ++waves_done;
             for (uint32_t j = 0; j < nj; ++j) {
                 const uint32_t id = hj[j].id;
@@ -558,6 +606,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         stats->tile_h = th;
         stats->n_active = n_sel;
         stats->bytes_per_visit = bytes_per_visit(tw, th, kind);
+        stats->solve_kernel_us = kern_us;
+        stats->solve_launches = kern_launches;
     }
     if (!converged) return fail(GHCM_ERR_NOT_CONVERGED, "wave cap %llu reached", (unsigned long long)max_waves);
     return GHCM_OK;
@@ -635,6 +685,14 @@ extern "C" int ghcm_dev_download(ghcm_ctx* ctx, void* dst, const void* src, size
     if (!ctx || (bytes && (!dst || !src))) return fail(GHCM_ERR_INVALID_ARG, "null argument");
     HIP_TRY(hipSetDevice(ctx->ordinal));
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_dev_copy(ghcm_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return GHCM_OK;
 }

@@ -8,7 +8,9 @@
 
 namespace ghcm {
 
-using solve_launcher_t = hipError_t (*)(const SolveArgs&, uint32_t grid, hipStream_t);
+// ev0/ev1: optional events stamped at the start and end of the dispatch.
+using solve_launcher_t = hipError_t (*)(const SolveArgs&, uint32_t grid, hipStream_t,
+                                        hipEvent_t ev0, hipEvent_t ev1);
 
 // Returns nullptr when no kernel is built for (tile_w, tile_h, field kind, mode).
 // mode: 0 systolic, 1 FIM.  host: job list given by the host scheduler.

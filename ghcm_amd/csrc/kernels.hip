@@ -25,6 +25,7 @@
 // list[cur^1] once per wave (atomicMax on qwave), sharded over kShards counters.
 // Host mode takes an explicit job list and writes one status record per job
 // (src/eikonal_hcm.cpp:396-420).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -578,9 +579,16 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
 }
 
 // ---- launcher table ----
+// ev0/ev1 (optional) are stamped by the dispatch itself (hipExtLaunchKernelGGL), so
+// timing a launch adds no marker packets to the stream.
 template <int TW, int TH, int KIND, int MODE, bool HOST>
-static hipError_t launch_solve_t(const SolveArgs& a, uint32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_solve<TW, TH, KIND, MODE, HOST>), dim3(grid), dim3(64), 0, s, a);
+static hipError_t launch_solve_t(const SolveArgs& a, uint32_t grid, hipStream_t s,
+                                 hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve<TW, TH, KIND, MODE, HOST>), dim3(grid), dim3(64), 0, s,
+                              ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL((k_solve<TW, TH, KIND, MODE, HOST>), dim3(grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

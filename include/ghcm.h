@@ -83,6 +83,8 @@ typedef struct ghcm_config {
     uint32_t max_waves;             /* safety cap on waves, 0 = automatic             */
     uint32_t waves_per_poll;        /* device scheduler: waves enqueued between host
                                        polls of the done flag, 0 = automatic          */
+    uint32_t profile_kernels;       /* 1: time every tile-solve launch with HIP events
+                                       (fills solve_kernel_us / solve_launches)       */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */
@@ -100,6 +102,8 @@ typedef struct ghcm_stats {
     uint32_t n_active;              /* n used                                         */
     uint32_t bytes_per_visit;       /* algorithmic HBM bytes of one tile visit        */
     uint32_t reserved;
+    double   solve_kernel_us;       /* profile_kernels: summed tile-solve launch time */
+    uint64_t solve_launches;        /* profile_kernels: tile-solve launches timed     */
 } ghcm_stats;
 
 /* Device description (OclDevInfo fields, src/util_opencl.h:4-16). */
@@ -153,6 +157,7 @@ int ghcm_dev_alloc(ghcm_ctx* ctx, size_t bytes, void** d_out);
 int ghcm_dev_free(ghcm_ctx* ctx, void* d_ptr);
 int ghcm_dev_upload(ghcm_ctx* ctx, void* d_dst, const void* src, size_t bytes);
 int ghcm_dev_download(ghcm_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int ghcm_dev_copy(ghcm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 
 /* ---- batched solve: many uneven instances in one device wave loop ---------- */
 typedef struct ghcm_instance {
