@@ -9,8 +9,11 @@ device wave loop until the active-tile list drains.  Field and time buffers are 
 in HBM before the timed region starts.
 
 Prints ONE JSON line on stdout: Mcells/s of the whole solve, the roofline entry of the
-dominant kernel (k_solve: algorithmic bytes per launch / average launch duration, HIP
-events on the solver stream), and the CPU baseline (the oracle's FP32 FMM on this host).
+dominant kernel (the tile-solve kernel: algorithmic bytes per launch / average launch
+duration, from HIP events stamped by each dispatch on the solver stream), and the CPU
+baseline (the oracle's FP32 FMM on this host).  Stamping every launch perturbs the loop
+(about 15 %), so `value` comes from the un-instrumented timed steps and the roofline from
+the same number of instrumented steps run right after them.
 """
 from __future__ import annotations
 
@@ -29,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 W = H = 4096
 FIELD_SEED, POINT_SEED = 20261202, 20261203
 METRIC = "Mcells/s at 4096² (% HBM peak); batched solves/s vs plain; p99 latency"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_c2_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_sweep_c2_traffic.txt")
 
 
 def log(*a):
@@ -79,9 +82,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tile", default="16x16")
-    ap.add_argument("--kernel", default="systolic", choices=["systolic", "fim"])
-    ap.add_argument("--n-active", type=int, default=16384)
+    ap.add_argument("--tile", default="64x64")
+    ap.add_argument("--kernel", default="sweep", choices=["systolic", "fim", "sweep"])
+    ap.add_argument("--n-active", type=int, default=512)
+    ap.add_argument("--inner-cap", type=int, default=1, help="in-tile rounds per visit (0 = converge)")
+    ap.add_argument("--waves-per-poll", type=int, default=8)
+    ap.add_argument("--launch-mode", type=int, default=0, help="0 per-wave launches, 1 persistent")
     ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -99,14 +105,15 @@ def main() -> int:
     import ghcm_amd as g
 
     tw, th = (int(v) for v in args.tile.split("x"))
-    kernel = g.KERNEL_FIM if args.kernel == "fim" else g.KERNEL_SYSTOLIC
+    kernel = {"fim": g.KERNEL_FIM, "systolic": g.KERNEL_SYSTOLIC, "sweep": g.KERNEL_SWEEP}[args.kernel]
     log(f"bench: generating {W}x{H} workload")
     speed = g.gen_smooth_speed(W, H, FIELD_SEED, 32.0, 0.1, 10.0)
     seeds = g.gen_points(W, H, POINT_SEED, 4)
     cost = g.cost_from_speed(speed)
 
     ctx = g.Context(0, tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
-                    profile_kernels=1)
+                    inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,
+                    launch_mode=args.launch_mode, profile_kernels=0)
     info = g.device_info(0)
     d_cost = ctx.to_device(cost)
     d_time = ctx.alloc(cost.shape, np.float32)
@@ -135,9 +142,13 @@ def main() -> int:
     log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.1f} Mcells/s")
 
     t_gpu = d_time.download()
-    launches = sum(s["solve_launches"] for s in sts)
-    kern_us = sum(s["solve_kernel_us"] for s in sts)
-    visits = sum(s["tile_visits"] for s in sts)
+    # Roofline of the tile-solve kernel over the same number of instrumented steps.
+    ctx.set_config(profile_kernels=1)
+    psts = [step() for _ in range(args.steps)]
+    ctx.set_config(profile_kernels=0)
+    launches = sum(s["solve_launches"] for s in psts)
+    kern_us = sum(s["solve_kernel_us"] for s in psts)
+    visits = sum(s["tile_visits"] for s in psts)
     bpv = sts[0]["bytes_per_visit"]
     bytes_per_launch = visits * bpv / max(launches, 1)
     avg_launch_us = kern_us / max(launches, 1)
@@ -146,9 +157,12 @@ def main() -> int:
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None if traffic is None else round(traffic, 1),
-            "kernel": "k_solve", "avg_launch_us": round(avg_launch_us, 3),
+            "kernel": "k_solve_sweep" if kernel == g.KERNEL_SWEEP else "k_solve",
+            "avg_launch_us": round(avg_launch_us, 3),
             "bytes_per_launch": round(bytes_per_launch, 1),
-            "launches_per_step": launches / args.steps}
+            "launches_per_step": launches / args.steps,
+            "ms_per_step_instrumented": round(
+                statistics.mean(s["duration_total_us"] for s in psts) / 1e3, 4)}
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcells/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -157,7 +171,8 @@ def main() -> int:
            "config": {"workload": f"plain gHCM {W}x{H} FP32 cost, smooth speed [0.1,10] "
                                   f"sigma 32, 4 point sources",
                       "tile": f"{tw}x{th}", "kernel": args.kernel,
-                      "n_active": args.n_active, "scheduler": "device_topn",
+                      "n_active": args.n_active, "inner_rounds_cap": args.inner_cap,
+                      "launch_mode": args.launch_mode, "scheduler": "device_topn",
                       "fim_minimum_change": 1e-5, "parallelism": "single_gpu",
                       "device": info["arch"]},
            "solve": {"waves": sts[0]["waves"], "tile_visits": sts[0]["tile_visits"],

@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libghcm.so")
 OK, ERR_INVALID_ARG, ERR_NO_DEVICE, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_CONVERGED, \
     ERR_UNSUPPORTED, ERR_OVER_BUDGET, ERR_NOT_FOUND, ERR_STOPPED = range(10)
 # ghcm_kernel / ghcm_scheduler / ghcm_field_kind
-KERNEL_SYSTOLIC, KERNEL_FIM = 0, 1
+KERNEL_SYSTOLIC, KERNEL_FIM, KERNEL_SWEEP = 0, 1, 2
 SCHED_DEVICE_TOPN, SCHED_HOST_PQ = 0, 1
 FIELD_COST_F32, FIELD_SPEED_F32, FIELD_SPEED_F16, FIELD_SPEED_BF16 = 0, 1, 2, 3
 
@@ -46,7 +46,7 @@ class Config(ctypes.Structure):
                 ("cache_mode", ctypes.c_int32), ("kernel", ctypes.c_int32),
                 ("scheduler", ctypes.c_int32), ("inner_iter_cap", ctypes.c_uint32),
                 ("max_waves", ctypes.c_uint32), ("waves_per_poll", ctypes.c_uint32),
-                ("profile_kernels", ctypes.c_uint32)]
+                ("profile_kernels", ctypes.c_uint32), ("launch_mode", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):

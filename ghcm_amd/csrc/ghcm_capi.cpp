@@ -107,10 +107,10 @@ struct ghcm_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    hipEvent_t ev_poll[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> ev_prof;    // profile_kernels: begin/end pairs, 2 chunks
+    std::vector<hipEvent_t> ev_prof;    // profile_kernels: begin/end pairs per queued wave
+    std::vector<hipEvent_t> ev_slot;    // device scheduler: one per chunk in flight
     // device scratch
-    DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals;
+    DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals, sel;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
     PinnedBuf h_ctrl, h_init, h_seeds, h_inst, h_jobs, h_status;
 };
@@ -157,6 +157,7 @@ extern "C" void ghcm_config_default(ghcm_config* c) {
     c->max_waves = 0;
     c->waves_per_poll = 0;
     c->profile_kernels = 0;
+    c->launch_mode = 0;
 }
 
 extern "C" int ghcm_device_count(int* count) {
@@ -199,7 +200,8 @@ static int check_config(const ghcm_config* c) {
     if (!c) return fail(GHCM_ERR_INVALID_ARG, "config is null");
     if (!(c->min_change >= 0.0f) || !std::isfinite(c->min_change))
         return fail(GHCM_ERR_INVALID_ARG, "min_change must be finite and >= 0");
-    if (c->kernel != GHCM_KERNEL_SYSTOLIC && c->kernel != GHCM_KERNEL_FIM)
+    if (c->kernel != GHCM_KERNEL_SYSTOLIC && c->kernel != GHCM_KERNEL_FIM &&
+        c->kernel != GHCM_KERNEL_SWEEP)
         return fail(GHCM_ERR_INVALID_ARG, "unknown kernel %d", c->kernel);
     if (c->scheduler != GHCM_SCHED_DEVICE_TOPN && c->scheduler != GHCM_SCHED_HOST_PQ)
         return fail(GHCM_ERR_INVALID_ARG, "unknown scheduler %d", c->scheduler);
@@ -208,8 +210,9 @@ static int check_config(const ghcm_config* c) {
     if (!(c->compute_unit_overload > 0.0f) || !(c->workers_overload > 0.0f))
         return fail(GHCM_ERR_INVALID_ARG, "overloads must be > 0");
     for (int k = 0; k < 4; ++k)
-        if (!get_solve_launcher((int)c->tile_w, (int)c->tile_h, k, c->kernel, false))
-            return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u", c->tile_w, c->tile_h);
+        if (!get_solve_kernel((int)c->tile_w, (int)c->tile_h, k, c->kernel, false).launch)
+            return fail(GHCM_ERR_UNSUPPORTED, "no kernel %d for tile %ux%u", c->kernel, c->tile_w,
+                        c->tile_h);
     return GHCM_OK;
 }
 
@@ -233,8 +236,6 @@ extern "C" int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** o
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_begin);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_end);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_poll[0], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_poll[1], hipEventDisableTiming);
     if (e != hipSuccess) {
         ghcm_ctx_destroy(ctx);
         return fail(GHCM_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
@@ -250,7 +251,7 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->inst, &ctx->tinst, &ctx->qwave, &ctx->key, &ctx->mask, &ctx->tmin,
                       &ctx->list, &ctx->ctrl, &ctx->seeds, &ctx->jobs, &ctx->status,
-                      &ctx->tstat, &ctx->totals,
+                      &ctx->tstat, &ctx->totals, &ctx->sel,
                       &ctx->field_io, &ctx->time_io})
         b->release();
     for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_init, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
@@ -258,9 +259,9 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
         b->release();
     if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
-    for (hipEvent_t ev : ctx->ev_poll)
-        if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : ctx->ev_prof)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : ctx->ev_slot)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -308,7 +309,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     if (n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
     const uint32_t tw = cfg.tile_w, th = cfg.tile_h;
     const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
-    solve_launcher_t launch = get_solve_launcher((int)tw, (int)th, kind, cfg.kernel, host_sched);
+    const SolveKernel sk = get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched);
+    const solve_launcher_t launch = sk.launch;
     if (!launch)
         return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u kind %d kernel %d%s", tw, th,
                     kind, cfg.kernel, host_sched ? " (host scheduler)" : "");
@@ -359,7 +361,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     if ((rc = ctx->seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_init.reserve(sizeof(GhcmCtrl)))) return rc;
-    if ((rc = ctx->h_ctrl.reserve(2 * sizeof(GhcmCtrl) + 64))) return rc;
+    if ((rc = ctx->h_ctrl.reserve(4 * sizeof(GhcmCtrl) + 64))) return rc;
 
     uint32_t* hs = ctx->h_seeds.as<uint32_t>();
     {
@@ -373,6 +375,11 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     }
     uint32_t n_sel = cfg.n_active ? cfg.n_active
                                   : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * ctx->cus));
+    if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, std::min<uint64_t>(n_sel, T))))) return rc;
+    // Grid of the device-mode solve kernel: at most one resident round of work-groups;
+    // each work-group loops over list entries, so the grid never has to match the list.
+    const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
+    const uint32_t grid = std::max(1u, std::min(n_sel, resident));
     GhcmCtrl* init = ctx->h_init.as<GhcmCtrl>();
     std::memset(init, 0, sizeof *init);
     init->n_select = n_sel;
@@ -389,6 +396,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     a.tiles = T;
     a.shard_cap = shard_cap;
     a.tstat = ctx->tstat.as<uint32_t>();
+    a.jobs = ctx->sel.as<uint32_t>();
     a.ctrl = ctx->ctrl.as<GhcmCtrl>();
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
@@ -413,27 +421,67 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0, kern_launches = 0;
     double kern_us = 0.0;
 
-    if (!host_sched) {
-        // ---- device-resident wave loop; the host only polls a done flag ----
+    const PersistentKernel pk = get_persistent_kernel((int)tw, (int)th, kind, cfg.kernel);
+    const bool persistent = !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
+    if (persistent) {
+        // ---- one cooperative launch runs every wave (grid = one resident round) ----
+        const int per_cu = pk.blocks_per_cu();
+        uint32_t pgrid = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, ctx->cus);
+        pgrid = std::min<uint32_t>(pgrid, kBarGroup * kMaxBarGroups);
+        pgrid = std::max<uint32_t>(1, std::min<uint32_t>(pgrid, std::max<uint32_t>(n_sel, 1)));
         HIP_TRY(launch_wave_end(a, 1u, st));
+        if (cfg.profile_kernels) {
+            while (ctx->ev_prof.size() < 2) {
+                hipEvent_t ev;
+                HIP_TRY(hipEventCreate(&ev));
+                ctx->ev_prof.push_back(ev);
+            }
+            HIP_TRY(hipEventRecord(ctx->ev_prof[0], st));
+        }
+        HIP_TRY(pk.launch(a, pgrid, st));
+        if (cfg.profile_kernels) HIP_TRY(hipEventRecord(ctx->ev_prof[1], st));
+        HIP_TRY(hipEventRecord(ctx->ev_end, st));
         GhcmCtrl* hc = ctx->h_ctrl.as<GhcmCtrl>();
-        uint32_t chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
-        const uint32_t chunk_max = cfg.waves_per_poll ? cfg.waves_per_poll : 64;
+        HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (cfg.profile_kernels) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_prof[0], ctx->ev_prof[1]));
+            kern_us += 1000.0 * ms;
+            kern_launches += 1;
+        }
+        if (hc[0].abort) return fail(GHCM_ERR_HIP, "device wave loop aborted: a grid wait timed out");
+        converged = hc[0].done != 0;
+        waves_done = hc[0].busy_waves;
+    } else if (!host_sched) {
+        // ---- device-resident wave loop; the host only polls a done flag ----
+        // Fixed chunks of `chunk` waves, up to kSlots chunks in flight: the GPU always has
+        // queued work while the host polls, and after the last busy wave at most
+        // kSlots * chunk no-op waves (each a cheap early-exit launch) remain queued.
+        HIP_TRY(launch_wave_end(a, 1u, st));
+        constexpr int kSlots = 4;
+        GhcmCtrl* hc = ctx->h_ctrl.as<GhcmCtrl>();
+        const uint32_t chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
         const bool prof = cfg.profile_kernels != 0;
+        while (ctx->ev_slot.size() < (size_t)kSlots) {
+            hipEvent_t ev;
+            HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            ctx->ev_slot.push_back(ev);
+        }
         if (prof) {
-            const size_t need = 2ull * 2 * chunk_max;
+            const size_t need = 2ull * kSlots * chunk;
             while (ctx->ev_prof.size() < need) {
                 hipEvent_t ev;
                 HIP_TRY(hipEventCreate(&ev));
                 ctx->ev_prof.push_back(ev);
             }
         }
-        uint32_t chunk_len[2] = {0, 0};
+        uint32_t chunk_len[kSlots] = {0, 0, 0, 0};
         auto harvest = [&](int sl) -> int {        // add the slot's launch times
             for (uint32_t k = 0; k < chunk_len[sl]; ++k) {
                 float ms = 0.0f;
-                hipEvent_t b = ctx->ev_prof[(size_t)(sl * chunk_max + k) * 2];
-                hipEvent_t e = ctx->ev_prof[(size_t)(sl * chunk_max + k) * 2 + 1];
+                hipEvent_t b = ctx->ev_prof[((size_t)sl * chunk + k) * 2];
+                hipEvent_t e = ctx->ev_prof[((size_t)sl * chunk + k) * 2 + 1];
                 HIP_TRY(hipEventElapsedTime(&ms, b, e));
                 kern_us += 1000.0 * ms;
                 ++kern_launches;
@@ -442,36 +490,36 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             return GHCM_OK;
         };
         uint64_t enqueued = 0;
-        int inflight = 0, slot = 0;
-        // Two chunks in flight: the GPU never waits on the host's poll.
+        int inflight = 0, head = 0;                 // head: next slot to fill
         for (;;) {
             for (uint32_t k = 0; k < chunk; ++k) {
                 if (prof)
-                    HIP_TRY(launch(a, n_sel, st, ctx->ev_prof[(size_t)(slot * chunk_max + k) * 2],
-                                   ctx->ev_prof[(size_t)(slot * chunk_max + k) * 2 + 1]));
+                    HIP_TRY(launch(a, grid, st, ctx->ev_prof[((size_t)head * chunk + k) * 2],
+                                   ctx->ev_prof[((size_t)head * chunk + k) * 2 + 1]));
                 else
-                    HIP_TRY(launch(a, n_sel, st, nullptr, nullptr));
+                    HIP_TRY(launch(a, grid, st, nullptr, nullptr));
                 HIP_TRY(launch_wave_end(a, 0u, st));
             }
-            chunk_len[slot] = prof ? chunk : 0;
+            chunk_len[head] = prof ? chunk : 0;
             enqueued += chunk;
-            HIP_TRY(hipMemcpyAsync(&hc[slot], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipEventRecord(ctx->ev_poll[slot], st));
+            HIP_TRY(hipMemcpyAsync(&hc[head], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(ctx->ev_slot[head], st));
             ++inflight;
-            slot ^= 1;
-            if (inflight == 2) {
-                HIP_TRY(hipEventSynchronize(ctx->ev_poll[slot]));
-                if ((rc = harvest(slot))) return rc;
+            head = (head + 1) % kSlots;
+            if (inflight == kSlots) {
+                const int oldest = head;            // the slot filled kSlots chunks ago
+                HIP_TRY(hipEventSynchronize(ctx->ev_slot[oldest]));
+                if ((rc = harvest(oldest))) return rc;
                 --inflight;
-                if (hc[slot].done) { converged = true; break; }
+                if (hc[oldest].done) { converged = true; break; }
             }
             if (enqueued >= max_waves) break;
-            if (chunk < chunk_max) chunk *= 2;
         }
         HIP_TRY(hipEventRecord(ctx->ev_end, st));
         HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        if ((rc = harvest(0)) || (rc = harvest(1))) return rc;
+        for (int sl = 0; sl < kSlots; ++sl)
+            if ((rc = harvest(sl))) return rc;
         converged = hc[0].done != 0;
         waves_done = hc[0].busy_waves;
     } else {
@@ -480,7 +528,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         if ((rc = ctx->status.reserve(sizeof(HostStatus) * n_sel))) return rc;
         if ((rc = ctx->h_jobs.reserve(sizeof(HostJob) * n_sel))) return rc;
         if ((rc = ctx->h_status.reserve(sizeof(HostStatus) * n_sel))) return rc;
-        a.jobs = ctx->jobs.as<HostJob>();
+        a.host_jobs = ctx->jobs.as<HostJob>();
         a.status = ctx->status.as<HostStatus>();
         HostJob* hj = ctx->h_jobs.as<HostJob>();
         HostStatus* hst = ctx->h_status.as<HostStatus>();
@@ -544,7 +592,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
                 dirty[id] = 0;   // cleared before results, so same-wave neighbours re-queue
                 ++nj;
             }
-            a.n_jobs = nj;
+            a.n_host_jobs = nj;
             HIP_TRY(hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * nj, hipMemcpyHostToDevice, st));
             if (cfg.profile_kernels)
                 HIP_TRY(launch(a, nj, st, ctx->ev_prof[0], ctx->ev_prof[1]));
@@ -582,7 +630,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
 
     // Per-tile counters summed once, outside the timed span.
     HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
-    unsigned long long* ht = reinterpret_cast<unsigned long long*>(ctx->h_ctrl.as<char>() + 2 * sizeof(GhcmCtrl));
+    unsigned long long* ht = reinterpret_cast<unsigned long long*>(ctx->h_ctrl.as<char>() + 4 * sizeof(GhcmCtrl));
     HIP_TRY(hipMemcpyAsync(ht, ctx->totals.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     visits = ht[0];

@@ -42,21 +42,28 @@ struct GhcmInst {
 constexpr uint32_t kShards = 32;        // list shards
 constexpr uint32_t kCounterStride = 32; // u32 between shard counters (128 B)
 
+constexpr uint32_t kBarGroup = 32;      // work-groups per arrival counter (persistent loop)
+constexpr uint32_t kMaxBarGroups = 64;
+
 struct GhcmCtrl {
     uint32_t wave;          // index of the wave being (or about to be) processed
     uint32_t cur;           // list[cur] holds the tiles queued for `wave`
     uint32_t done;          // 1 once a wave starts with an empty list
     uint32_t n_select;      // n
-    uint32_t thr_lo;        // keys < thr_lo are selected
-    uint32_t thr_hi;        // keys in [thr_lo, thr_hi) are selected up to `quota`
-    uint32_t quota;
-    uint32_t quota_used;
+    uint32_t n_jobs;        // tiles selected for `wave`
+    uint32_t jobs_from_list;// 1: the jobs are list[cur] itself; 0: jobs[] (compacted top-n)
     uint32_t busy_waves;    // waves that had work
     uint32_t total;         // entries in list[cur]
-    uint32_t pad0[6];
+    uint32_t wave_ready;    // persistent loop: wave index whose selection is published
+    uint32_t abort;         // persistent loop: a barrier wait timed out
+    uint32_t carried;       // tiles carried over by the top-n selection (all waves)
+    uint32_t pad0[5];
     uint32_t prefix[kShards + 1];   // exclusive prefix of the shard counts of list[cur]
     uint32_t pad1[31];
     uint32_t count[2][kShards * kCounterStride];
+    uint32_t arrive_top;
+    uint32_t pad2[31];
+    uint32_t arrive[kMaxBarGroups * kCounterStride];
 };
 
 // Host-scheduled wave: job and status records (src/eikonal_hcm.cpp:22-32,
@@ -87,13 +94,14 @@ struct SolveArgs {
     uint32_t        tiles;
     uint32_t        shard_cap;
     uint32_t*       tstat;   // [3][tiles]
+    uint32_t*       jobs;    // compacted top-n selection, capacity min(n, tiles)
     GhcmCtrl*       ctrl;
     float           eps;
     uint32_t        iter_cap;
     // host-scheduled mode
-    const HostJob*  jobs;
+    const HostJob*  host_jobs;
     HostStatus*     status;
-    uint32_t        n_jobs;
+    uint32_t        n_host_jobs;
 };
 
 }  // namespace ghcm

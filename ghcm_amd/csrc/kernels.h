@@ -12,9 +12,20 @@ namespace ghcm {
 using solve_launcher_t = hipError_t (*)(const SolveArgs&, uint32_t grid, hipStream_t,
                                         hipEvent_t ev0, hipEvent_t ev1);
 
-// Returns nullptr when no kernel is built for (tile_w, tile_h, field kind, mode).
-// mode: 0 systolic, 1 FIM.  host: job list given by the host scheduler.
-solve_launcher_t get_solve_launcher(int tile_w, int tile_h, int kind, int mode, bool host);
+struct SolveKernel {
+    solve_launcher_t launch;          // nullptr when no kernel is built for the combination
+    int (*blocks_per_cu)();           // resident work-groups per CU (occupancy)
+};
+
+// mode: 0 systolic, 1 FIM, 2 sweep.  host: job list given by the host scheduler.
+SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host);
+
+// Persistent device wave loop (cooperative launch); null when not built for the combination.
+struct PersistentKernel {
+    hipError_t (*launch)(const SolveArgs&, uint32_t grid, hipStream_t);
+    int (*blocks_per_cu)();
+};
+PersistentKernel get_persistent_kernel(int tile_w, int tile_h, int kind, int mode);
 
 hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);

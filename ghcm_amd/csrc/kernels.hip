@@ -55,22 +55,18 @@ __device__ __forceinline__ float load_cost(const void* f, size_t i) {
 // 296-297); otherwise T = m + (d + sqrt((sqrt2 c - d)(sqrt2 c + d)))/2, the root of
 // (T-a)^2 + (T-b)^2 = c^2 (:299-300) with the discriminant factored and the large
 // common part m added once.  Result min'ed with t (:305-308).
+// Branch-free: both candidates are formed and one is selected.  `!(c >= d)` also takes
+// the 1-D branch when d is NaN (a = b = +INF), which yields +INF.  The square root is the
+// hardware v_sqrt_f32 (1 ulp); its argument lies in [c^2, 2c^2].
 __device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
-    float m = fminf(ax, ay);
-    float M = fmaxf(ax, ay);
-    float r;
-    if (__builtin_isinf(M)) {
-        r = m + c;
-    } else {
-        float d = M - m;
-        if (c < d) {
-            r = m + c;
-        } else {
-            float s = 1.41421356237309515f * c;
-            float q = (s - d) * (s + d);
-            r = m + 0.5f * (d + __builtin_sqrtf(q));
-        }
-    }
+    const float m = fminf(ax, ay);
+    const float M = fmaxf(ax, ay);
+    const float d = M - m;
+    const float s = 1.41421356237309515f * c;
+    float q = (s - d) * (s + d);
+    q = q > 0.0f ? q : 0.0f;
+    const float two = m + 0.5f * (d + __builtin_amdgcn_sqrtf(q));
+    const float r = (c >= d) ? two : m + c;
     return fminf(t, r);
 }
 
@@ -116,13 +112,63 @@ __device__ __forceinline__ uint32_t list_entry(const SolveArgs& a, const uint32_
     return a.list[((size_t)cur * kShards + lo) * a.shard_cap + (i - prefix[lo])];
 }
 
+// Stage a PW x PH box of time and cost (the tile plus its halo ring, origin (ox-1, oy-1))
+// into LDS; out-of-grid cells read +INF.  Every global load of a thread is issued before
+// the first LDS store, so the box costs one memory round trip rather than one per row.
+// Job j of the current wave: entry j of list[cur] when every queued tile was selected,
+// else entry j of the compacted top-n selection.
+__device__ __forceinline__ uint32_t job_tile(const SolveArgs& a, const uint32_t* prefix,
+                                             uint32_t cur, uint32_t from_list, uint32_t j) {
+    return from_list ? list_entry(a, prefix, cur, j) : a.jobs[j];
+}
+
+// Claim a job's activation record: its border bits (and open-all flag) for this wave;
+// the key and bits are reset for the tile's next activation.
+__device__ __forceinline__ uint32_t claim_tile(const SolveArgs& a, uint32_t g, uint32_t p) {
+    uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
+    uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
+    const uint32_t mb = *mp;
+    *kp = kInfBits;
+    *mp = 0u;
+    return mb;
+}
+
+template <int PW, int PH, int NT, int KIND>
+__device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, int tid,
+                                           float* t_s, float* c_s) {
+    constexpr int PP = PW * PH;
+    constexpr int NL = (PP + NT - 1) / NT;
+    const int W = (int)in.w, H = (int)in.h;
+    float tv[NL], cv[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = tid + NT * k;
+        const int bx = i % PW, by = i / PW;
+        const int gx = ox + bx - 1, gy = oy + by - 1;
+        tv[k] = __builtin_inff();
+        cv[k] = __builtin_inff();
+        if (i < PP && gx >= 0 && gy >= 0 && gx < W && gy < H) {
+            const size_t idx = (size_t)gy * (size_t)W + (size_t)gx;
+            tv[k] = in.time[idx];
+            cv[k] = load_cost<KIND>(in.field, idx);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+        const int i = tid + NT * k;
+        if (i < PP) {
+            t_s[i] = tv[k];
+            c_s[i] = cv[k];
+        }
+    }
+}
+
 template <int TW, int TH, int KIND, int MODE, bool HOST>
 __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uint32_t wave,
                              uint32_t cur, uint32_t p, uint32_t job_slot,
                              float* t_s, float* c_s, uint8_t* open_s, uint8_t* add_s) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
-    constexpr int PP = PW * PH;
     constexpr int P = TW * TH;
     constexpr int CPL = (P + 63) / 64;
     const int lane = threadIdx.x;
@@ -137,18 +183,7 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
     const int sx = min(TW, W - ox), sy = min(TH, H - oy);
 
     // ---- stage tile + halo ring into LDS (out of grid -> +INF) ----
-    for (int i = lane; i < PP; i += 64) {
-        int bx = i % PW, by = i / PW;
-        int gx = ox + bx - 1, gy = oy + by - 1;
-        float t = __builtin_inff(), c = __builtin_inff();
-        if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            size_t idx = (size_t)gy * (size_t)W + (size_t)gx;
-            t = in.time[idx];
-            c = load_cost<KIND>(in.field, idx);
-        }
-        t_s[i] = t;
-        c_s[i] = c;
-    }
+    stage_tile<PW, PH, 64, KIND>(in, ox, oy, lane, t_s, c_s);
     __syncthreads();
 
     float told[CPL];
@@ -391,61 +426,345 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 
     if constexpr (HOST) {
         const uint32_t j = blockIdx.x;
-        if (j >= a.n_jobs) return;
-        const HostJob job = a.jobs[j];
+        if (j >= a.n_host_jobs) return;
+        const HostJob job = a.host_jobs[j];
         process_tile<TW, TH, KIND, MODE, true>(a, job.id, job.border, 0u, 0u, 0u, j,
                                                t_s, c_s, open_s, add_s);
     } else {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
         if (ctrl->done) return;
-        const uint32_t c = ctrl->total;
-        if (blockIdx.x >= c) return;
+        const uint32_t nj = ctrl->n_jobs;
+        if (blockIdx.x >= nj) return;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
-        const uint32_t thr_lo = ctrl->thr_lo, thr_hi = ctrl->thr_hi, quota = ctrl->quota;
+        const uint32_t from_list = ctrl->jobs_from_list;
         const uint32_t p = wave & 1u;
         for (int i = lane; i <= (int)kShards; i += 64) s_prefix[i] = ctrl->prefix[i];
         __syncthreads();
-        for (uint32_t i = blockIdx.x; i < c; i += gridDim.x) {
-            const uint32_t g = list_entry(a, s_prefix, cur, i);
-            uint32_t sel = 0, mb = 0;
-            if (lane == 0) {
-                uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
-                uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
-                uint32_t k = *kp;
-                sel = (k < thr_lo) ? 1u : 0u;
-                if (!sel && k < thr_hi) sel = (atomicAdd(&a.ctrl->quota_used, 1u) < quota) ? 1u : 0u;
-                mb = *mp;
-                *kp = kInfBits;
-                *mp = 0u;
-                if (!sel) activate(a, g, wave, cur, p, mb, __uint_as_float(k));  // carry
-            }
-            sel = __shfl(sel, 0);
+        for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
+            const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+            uint32_t mb = 0;
+            if (lane == 0) mb = claim_tile(a, g, p);
             mb = __shfl(mb, 0);
-            if (sel) {
-                process_tile<TW, TH, KIND, MODE, false>(a, g, mb, wave, cur, p, i,
-                                                        t_s, c_s, open_s, add_s);
-                __syncthreads();
-            }
+            process_tile<TW, TH, KIND, MODE, false>(a, g, mb, wave, cur, p, j,
+                                                    t_s, c_s, open_s, add_s);
+            __syncthreads();
         }
     }
 }
 
-// Closes the previous wave (swap lists), computes the shard prefix of the new list and,
-// when it holds more than n tiles, a threshold selecting its n lowest keys: a two-level
-// radix histogram over key bits [30:19] then [18:7] (keys are non-negative floats, so
-// their bit patterns order like the values).  One work-group of 1024 threads.
-__global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) {
-    __shared__ uint32_t hist[4096];
-    __shared__ uint32_t part[1024];
-    __shared__ uint32_t s_prefix[kShards + 1];
-    __shared__ uint32_t s_bin, s_below;
+// ---------------------------------------------------------------------------------------
+// Sweep tile solver: four wave64s per work-group relax one TW x TH tile (TW <= 64) with
+// the four Gauss-Seidel orders of fast sweeping (Zhao 2005, the method
+// data/eikonal_common.cl:227 cites), one order per wave, concurrently on the same LDS tile.
+// Lane L owns column x (x = L for sweeps towards +x, TW-1-L towards -x) and handles row
+// index s-L at step s (a skewed anti-diagonal front), so the upwind column neighbour of
+// the current row was produced by lane L-1 one step earlier: it arrives through a
+// wave_shr:1 DPP move, and the upwind row neighbour is the lane's own previous result.
+// The other neighbours are read from LDS (any value there is an upper bound, possibly
+// already lowered by another wave), and results are merged with LDS atomicMin.  A round is
+// the four sweeps; rounds repeat until no cell of the tile improves by >= eps.
+// ---------------------------------------------------------------------------------------
+constexpr int kSweepThreads = 256;
+constexpr uint32_t kSweepHardRounds = 64;
+
+__device__ __forceinline__ float dpp_from_lower_lane(float v) {
+    // wave_shr:1 -- lane i receives lane i-1; lane 0 receives +INF.
+    return __int_as_float(__builtin_amdgcn_update_dpp((int)kInfBits, __float_as_int(v), 0x138,
+                                                      0xF, 0xF, false));
+}
+
+// LDS words of a sweep tile: time box, a sink word, cost box.
+template <int TW, int TH>
+constexpr int sweep_lds_words() { return 2 * (TW + 2) * (TH + 2) + 1; }
+
+template <int TW, int TH, int KIND, bool HOST>
+__device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
+                           uint32_t p, uint32_t job_slot, float* sh,
+                           uint32_t* s_u32, float* s_f32) {
+    constexpr int PW = TW + 2;
+    constexpr int PH = TH + 2;
+    constexpr int PP = PW * PH;
+    constexpr int CB = PP + 1;                     // cost box offset: t_s | sink | c_s
+    float* const t_s = sh;
+    float* const c_s = sh + CB;
+    constexpr int P = TW * TH;
+    constexpr int CPT = (P + kSweepThreads - 1) / kSweepThreads;
+    static_assert(TW <= 64, "one lane per column");
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, L = tid & 63;
+
+    const uint32_t ii = (a.n_inst == 1) ? 0u : a.tinst[g];
+    const GhcmInst in = a.inst[ii];
+    const uint32_t lt = g - in.tile_base;
+    const int tx = (int)(lt % in.tiles_x);
+    const int ty = (int)(lt / in.tiles_x);
+    const int ox = tx * TW, oy = ty * TH;
+    const int W = (int)in.w, H = (int)in.h;
+    const int sx = min(TW, W - ox), sy = min(TH, H - oy);
+
+    stage_tile<PW, PH, kSweepThreads, KIND>(in, ox, oy, tid, t_s, c_s);
+    __syncthreads();
+    float told[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        int i = tid + kSweepThreads * k;
+        told[k] = (i < P) ? t_s[(i / TW + 1) * PW + (i % TW) + 1] : __builtin_inff();
+    }
+
+    const float eps = a.eps;
+    const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kSweepHardRounds) ? kSweepHardRounds : a.iter_cap;
+    const int dx = (w & 1) ? -1 : 1;
+    const int inc = (w & 2) ? -PW : PW;            // row step of this wave's sweep
+    const int x = (dx > 0) ? L : TW - 1 - L;
+    const bool lane_on = L < TW;
+    // Cell of this lane at step s: row index yi = s - L (rows ascending for inc > 0,
+    // descending otherwise).  b0 is the top-left corner of its 3x3 neighbourhood at s = 0;
+    // all six reads of a step are immediate offsets from one corner index.
+    const int q0 = ((inc > 0) ? (1 - L) : (TH + L)) * PW + (x + 1);
+    const int b0 = q0 - PW - 1;
+    constexpr int kDummy = 0;                      // corner of padded cell (1,1): in range
+    constexpr int kSinkB = PP - (PW + 1);          // corner whose centre is the sink word
+    uint32_t rounds = 0;
+    bool unfinished = false;
+    // Step s reads only cells whose values are upper bounds whatever the interleaving, so the
+    // LDS reads of step s+1 are issued before step s is computed (software pipelining, two
+    // register sets so no copy waits on a read in flight): the dependent chain per step is
+    // then DPP + Godunov update only.  The min over each axis is symmetric, so the reads do
+    // not depend on the sweep direction; the upwind values also arrive fresh through DPP
+    // (column) and `prev` (row).  Steps past the last row are inactive for every lane.
+    constexpr int S = TW + TH - 1;
+    constexpr int SE = (S + 1) & ~1;
+    struct Nb {
+        float tc, cc, tl, tr, tu, td;
+        int b;
+        bool act;
+    };
+    auto load = [&](int st, Nb& o) {
+        o.act = lane_on && (unsigned)(st - L) < (unsigned)TH;
+        o.b = b0 + st * inc;
+        const float* r = sh + (o.act ? o.b : kDummy);
+        o.tu = r[1];
+        o.tl = r[PW];
+        o.tc = r[PW + 1];
+        o.tr = r[PW + 2];
+        o.td = r[2 * PW + 1];
+        o.cc = r[CB + PW + 1];
+    };
+    for (;;) {
+        float prev = __builtin_inff();
+        bool changed = false;
+        auto step = [&](const Nb& n) {
+            const float left = dpp_from_lower_lane(prev);
+            const float ax = fminf(fminf(n.tl, left), n.tr);
+            const float ay = fminf(fminf(n.tu, prev), n.td);
+            const float nv = godunov(ax, ay, n.cc, n.tc);
+            // Plain store of an improvement.  Another wave may have lowered the cell since it
+            // was read, and this store can raise it back to a value that is still an upper
+            // bound; the round only ends when no sweep improves any cell, so a lost update
+            // costs at most another round.  Other lanes store into the sink word.
+            const bool imp = n.act & (nv < n.tc);
+            sh[(imp ? n.b : kSinkB) + PW + 1] = nv;
+            const float dec = n.tc - nv;                     // >= 0; NaN when both are INF
+            changed |= imp & (dec >= eps);
+            prev = n.act ? nv : __builtin_inff();
+        };
+        Nb A, B;
+        load(0, A);
+        for (int st = 0; st < SE; st += 2) {
+            load(st + 1, B);
+            __builtin_amdgcn_sched_barrier(0);
+            step(A);
+            load(st + 2, A);
+            __builtin_amdgcn_sched_barrier(0);
+            step(B);
+        }
+        ++rounds;
+        const bool any = __ballot(changed) != 0ull;
+        if (L == 0) s_u32[w] = any ? 1u : 0u;
+        __syncthreads();
+        const bool more = (s_u32[0] | s_u32[1] | s_u32[2] | s_u32[3]) != 0u;
+        __syncthreads();
+        if (!more) break;
+        if (rounds >= cap) { unfinished = true; break; }
+    }
+
+    // Border check, wave w takes side w: would the neighbour's adjacent cell improve by
+    // >= eps from this tile's edge (data/eikonal_ghcm_systolic.cl:8-41)?
+    {
+        const int side = w;
+        const bool has_nb = side == 0 ? tx > 0 : side == 1 ? ty > 0
+                          : side == 2 ? tx + 1 < (int)in.tiles_x : ty + 1 < (int)in.tiles_y;
+        bool flag = false;
+        float kc = __builtin_inff();
+        if (has_nb) {
+            const int len = (side & 1) ? sx : sy;
+            for (int j = L; j < len; j += 64) {
+                int bx, by;
+                if (side == 0)      { bx = 0;      by = j + 1; }
+                else if (side == 2) { bx = sx + 1; by = j + 1; }
+                else if (side == 1) { bx = j + 1;  by = 0; }
+                else                { bx = j + 1;  by = sy + 1; }
+                const int q = by * PW + bx;
+                float ax, ay;
+                if (side == 0)      { ax = t_s[q + 1];  ay = fminf(t_s[q - PW], t_s[q + PW]); }
+                else if (side == 2) { ax = t_s[q - 1];  ay = fminf(t_s[q - PW], t_s[q + PW]); }
+                else if (side == 1) { ay = t_s[q + PW]; ax = fminf(t_s[q - 1], t_s[q + 1]); }
+                else                { ay = t_s[q - PW]; ax = fminf(t_s[q - 1], t_s[q + 1]); }
+                const float t0 = t_s[q];
+                const float tn = godunov(ax, ay, c_s[q], t0);
+                if (tn < t0 && t0 - tn >= eps) {
+                    flag = true;
+                    kc = fminf(kc, tn);
+                }
+            }
+        }
+        const bool f = __ballot(flag) != 0ull;
+        kc = wave_min(kc);
+        if (L == 0) {
+            s_u32[4 + side] = f ? 1u : 0u;
+            s_f32[side] = kc;
+        }
+    }
+
+    // Commit improved cells (plain stores: one writer per tile per wave), tile min.
+    float tmin = __builtin_inff();
+    uint32_t improved = 0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = tid + kSweepThreads * k;
+        if (i < P) {
+            const int cx = i % TW, cy = i / TW;
+            if (cx < sx && cy < sy) {
+                const float tn = t_s[(cy + 1) * PW + cx + 1];
+                tmin = fminf(tmin, tn);
+                if (tn < told[k]) {
+                    in.time[(size_t)(oy + cy) * (size_t)W + (size_t)(ox + cx)] = tn;
+                    ++improved;
+                }
+            }
+        }
+    }
+    tmin = wave_min(tmin);
+    improved = wave_sum(improved);
+    if (L == 0) {
+        s_f32[4 + w] = tmin;
+        s_u32[8 + w] = improved;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const float tm = fminf(fminf(s_f32[4], s_f32[5]), fminf(s_f32[6], s_f32[7]));
+        const uint32_t imp = s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11];
+        a.tmin[g] = tm;
+        a.tstat[g] += 1u;
+        a.tstat[(size_t)a.tiles + g] += rounds;
+        a.tstat[2 * (size_t)a.tiles + g] += imp;
+        if constexpr (HOST) {
+            HostStatus st;
+            st.solves = rounds;
+            st.cells_updated = imp;
+            float sm = tm;
+            for (int sd = 0; sd < 4; ++sd) {
+                st.border[sd] = s_u32[4 + sd];
+                if (s_u32[4 + sd]) sm = fminf(sm, s_f32[sd]);
+            }
+            st.smallest_time = sm;
+            st.requeue = unfinished ? 1u : 0u;
+            a.status[job_slot] = st;
+        } else {
+            if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tm);
+        }
+    }
+    if constexpr (!HOST) {
+        if (tid < 4 && s_u32[4 + tid]) {
+            const int sd = tid;
+            const int nx = tx + (sd == 0 ? -1 : sd == 2 ? 1 : 0);
+            const int ny = ty + (sd == 1 ? -1 : sd == 3 ? 1 : 0);
+            const uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
+            activate(a, nb, wave, cur, p, 1u << (sd ^ 2), s_f32[sd]);
+        }
+    }
+}
+
+template <int TW, int TH, int KIND, bool HOST>
+__global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
+    __shared__ float sh[sweep_lds_words<TW, TH>()];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    const int tid = threadIdx.x;
+    if constexpr (HOST) {
+        const uint32_t j = blockIdx.x;
+        if (j >= a.n_host_jobs) return;
+        sweep_tile<TW, TH, KIND, true>(a, a.host_jobs[j].id, 0u, 0u, 0u, j, sh, s_u32, s_f32);
+    } else {
+        __shared__ uint32_t s_prefix[kShards + 1];
+        const GhcmCtrl* ctrl = a.ctrl;
+        if (ctrl->done) return;
+        const uint32_t nj = ctrl->n_jobs;
+        if (blockIdx.x >= nj) return;
+        const uint32_t wave = ctrl->wave;
+        const uint32_t cur = ctrl->cur;
+        const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t p = wave & 1u;
+        for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
+        __syncthreads();
+        for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
+            const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+            if (tid == 0) claim_tile(a, g, p);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32);
+            __syncthreads();
+        }
+    }
+}
+
+// Block-wide exclusive scan of v over NT threads (NT a multiple of 64, at most 1024;
+// part[] holds at least 64 + NT/64 words); returns the exclusive prefix of this thread
+// and stores the block total in *total.  Wave scans by __shfl_up, then one wave scans the
+// wave totals: two barriers instead of two per doubling step.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, uint32_t* total) {
+    constexpr int NWV = NT / 64;
+    static_assert(NT % 64 == 0 && NWV <= 64, "whole waves");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) part[wv] = x;
+    __syncthreads();
+    if (tid < 64) {
+        uint32_t t = tid < NWV ? part[tid] : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(t, off);
+            if (tid >= off) t += y;
+        }
+        if (tid < NWV) part[64 + tid] = t;
+    }
+    __syncthreads();
+    const uint32_t base = wv ? part[64 + wv - 1] : 0u;
+    *total = part[64 + NWV - 1];
+    __syncthreads();
+    return base + x - v;
+}
+
+// Wave boundary, run by one work-group of NT threads: empty the consumed list and swap
+// (unless `first`), compute the shard prefix of the new list and, when it holds more than
+// n tiles, select its n lowest keys: a two-level radix histogram over key bits [30:19]
+// then [18:7] (keys are non-negative floats, so their bit patterns order like the values)
+// gives the threshold bin; entries below it and, up to the quota, inside it are compacted
+// into jobs[]; the rest are carried to the next list with their key and border bits.
+template <int NT, int STAGE>
+__device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint32_t* part,
+                            uint32_t* s_prefix, uint32_t* s_misc, uint32_t* sg, uint32_t* sk) {
     GhcmCtrl* ctrl = a.ctrl;
     const int tid = threadIdx.x;
-    if (ctrl->done) return;
     uint32_t cur = ctrl->cur;
     const uint32_t wave0 = ctrl->wave;
+    __syncthreads();
     if (!first) {
         // list[cur] was consumed by the wave that just ran: empty it and swap.
         if (tid < (int)kShards) ctrl->count[cur][tid * kCounterStride] = 0u;
@@ -462,61 +781,204 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     __syncthreads();
     const uint32_t c = s_prefix[kShards];
     const uint32_t n = ctrl->n_select;
+    const uint32_t wave = wave0 + (first ? 0u : 1u);
+    const uint32_t p = wave & 1u;
     if (tid <= (int)kShards) ctrl->prefix[tid] = s_prefix[tid];
     if (tid == 0) {
-        if (!first) { ctrl->cur = cur; ctrl->wave = wave0 + 1u; }
+        ctrl->cur = cur;
+        ctrl->wave = wave;
         ctrl->total = c;
-        ctrl->quota_used = 0u;
         if (c == 0u) ctrl->done = 1u;
         else ctrl->busy_waves += 1u;
-        if (c <= n) { ctrl->thr_lo = 0xFFFFFFFFu; ctrl->thr_hi = 0xFFFFFFFFu; ctrl->quota = 0u; }
+        ctrl->jobs_from_list = 1u;
+        ctrl->n_jobs = c;
     }
-    if (c == 0u || c <= n) return;
-    const uint32_t p = (wave0 + (first ? 0u : 1u)) & 1u;
+    if (c <= n) return;
+
     const uint32_t* key = a.key + (size_t)p * a.tiles;
+    // Entries (tile, key) are read from the list once into LDS when they fit.
+    const bool staged = STAGE > 0 && c <= (uint32_t)STAGE;
+    if (staged) {
+        for (uint32_t i = tid; i < c; i += NT) {
+            const uint32_t g = list_entry(a, s_prefix, cur, i);
+            sg[i] = g;
+            sk[i] = key[g];
+        }
+        __syncthreads();
+    }
+    auto entry_g = [&](uint32_t i) { return staged ? sg[i] : list_entry(a, s_prefix, cur, i); };
+    auto entry_k = [&](uint32_t i, uint32_t g) { return staged ? sk[i] : key[g]; };
     uint32_t lo_base = 0u, below = 0u;
+    constexpr int BPT = 4096 / NT;                    // bins per thread
     for (int level = 0; level < 2; ++level) {
         const int shift = level == 0 ? 19 : 7;
-        for (int i = tid; i < 4096; i += 1024) hist[i] = 0u;
+        for (int i = tid; i < 4096; i += NT) hist[i] = 0u;
         __syncthreads();
-        for (uint32_t i = tid; i < c; i += 1024) {
-            uint32_t k = key[list_entry(a, s_prefix, cur, i)];
+        for (uint32_t i = tid; i < c; i += NT) {
+            const uint32_t k = entry_k(i, staged ? 0u : list_entry(a, s_prefix, cur, i));
             if (level == 1 && (k >> 19) != (lo_base >> 19)) continue;
             atomicAdd(&hist[(k >> shift) & 0xFFFu], 1u);
         }
         __syncthreads();
-        // Each thread owns 4 consecutive bins; exclusive scan of the 1024 partial sums.
-        const uint32_t h0 = hist[4 * tid], h1 = hist[4 * tid + 1], h2 = hist[4 * tid + 2],
-                       h3 = hist[4 * tid + 3];
-        part[tid] = h0 + h1 + h2 + h3;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            uint32_t v = tid >= off ? part[tid - off] : 0u;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
-        }
-        const uint32_t incl = part[tid] + below;
-        const uint32_t excl = incl - (h0 + h1 + h2 + h3);
-        if (excl < n && incl >= n) {           // the crossing bin is one of this thread's 4
-            uint32_t cum = excl, b = 4 * tid;
-            const uint32_t hh[4] = {h0, h1, h2, h3};
-            for (int j = 0; j < 4; ++j) {
-                if (cum + hh[j] >= n) { b = 4 * tid + j; break; }
-                cum += hh[j];
+        uint32_t mine = 0;
+        for (int j = 0; j < BPT; ++j) mine += hist[BPT * tid + j];
+        uint32_t tot;
+        const uint32_t excl = block_excl_scan<NT>(mine, part, &tot) + below;
+        if (excl < n && excl + mine >= n) {           // the crossing bin is one of mine
+            uint32_t cum = excl, b = BPT * tid;
+            for (int j = 0; j < BPT; ++j) {
+                const uint32_t h = hist[BPT * tid + j];
+                if (cum + h >= n) { b = BPT * tid + j; break; }
+                cum += h;
             }
-            s_bin = b;
-            s_below = cum;
+            s_misc[0] = b;
+            s_misc[1] = cum;
         }
         __syncthreads();
-        lo_base |= s_bin << shift;
-        below = s_below;
+        lo_base |= s_misc[0] << shift;
+        below = s_misc[1];
         __syncthreads();
     }
+    // Compaction: keys < lo_base (there are `below` of them) all go; keys in
+    // [lo_base, lo_base + 128) go while the quota n - below lasts.
+    const uint32_t quota = n - below;
+    uint32_t base = 0, quota_used = 0, carried = 0;
+    const uint32_t thr_hi = lo_base + (1u << 7);
+    for (uint32_t i0 = 0; i0 < c; i0 += NT) {
+        const uint32_t i = i0 + tid;
+        uint32_t g = 0, k = 0xFFFFFFFFu, low = 0, tie = 0;
+        if (i < c) {
+            g = entry_g(i);
+            k = entry_k(i, g);
+            low = k < lo_base ? 1u : 0u;
+            tie = (!low && k < thr_hi) ? 1u : 0u;
+        }
+        uint32_t tot_tie;
+        const uint32_t tie_rank = block_excl_scan<NT>(tie, part, &tot_tie) + quota_used;
+        const uint32_t sel = low | ((tie && tie_rank < quota) ? 1u : 0u);
+        uint32_t tot_sel;
+        const uint32_t pos = block_excl_scan<NT>(sel, part, &tot_sel);
+        if (i < c) {
+            if (sel) {
+                a.jobs[base + pos] = g;
+            } else {
+                // carry: move the activation record to the next parity, re-queue
+                uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
+                uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
+                const uint32_t mb = *mp;
+                const uint32_t kk = *kp;
+                *kp = kInfBits;
+                *mp = 0u;
+                activate(a, g, wave, cur, p, mb, __uint_as_float(kk));
+            }
+        }
+        base += tot_sel;
+        quota_used += tot_tie;
+        carried += (i0 + NT <= c ? NT : c - i0) - tot_sel;
+    }
     if (tid == 0) {
-        ctrl->thr_lo = lo_base;                 // `below` keys are < lo_base
-        ctrl->thr_hi = lo_base + (1u << 7);
-        ctrl->quota = n - below;
+        ctrl->jobs_from_list = 0u;
+        ctrl->n_jobs = base;
+        ctrl->carried += carried;
+    }
+}
+
+constexpr int kSelectStage = 4096;     // list entries staged in LDS by k_wave_end
+
+__global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) {
+    __shared__ uint32_t hist[4096];
+    __shared__ uint32_t part[128];
+    __shared__ uint32_t s_prefix[kShards + 1];
+    __shared__ uint32_t s_misc[2];
+    __shared__ uint32_t sg[kSelectStage];
+    __shared__ uint32_t sk[kSelectStage];
+    if (a.ctrl->done) return;
+    wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
+}
+
+// ---------------------------------------------------------------------------------------
+// Persistent device wave loop for the sweep solver: one cooperative launch runs every
+// wave.  Each work-group solves its share of the wave's jobs, then arrives at a
+// two-level counter (groups of kBarGroup work-groups, then a top counter); the last
+// arrival runs wave_select for the next wave with its whole work-group and publishes the
+// new wave index; the others wait for it.  Releases are agent-scope fences by every
+// thread before arriving; the waiter's acquire fence precedes its reads of the new wave.
+// A wait longer than ~2^26 polls sets `abort` and every work-group leaves the loop.
+// ---------------------------------------------------------------------------------------
+template <int TW, int TH, int KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a) {
+    constexpr int NW = sweep_lds_words<TW, TH>();
+    constexpr int TS = NW > 4096 + kSweepThreads ? NW : 4096 + kSweepThreads;
+    // The selection's histogram and scan buffer reuse the tile buffer: the work-group
+    // that runs wave_select has finished its tiles for the wave.
+    __shared__ float sh[TS];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    __shared__ uint32_t s_prefix[kShards + 1];
+    __shared__ uint32_t s_misc[4];
+    uint32_t* hist = reinterpret_cast<uint32_t*>(sh);
+    uint32_t* part = reinterpret_cast<uint32_t*>(sh) + 4096;
+    GhcmCtrl* ctrl = a.ctrl;
+    const int tid = threadIdx.x;
+    const uint32_t G = gridDim.x;
+    const uint32_t n_groups = (G + kBarGroup - 1) / kBarGroup;
+    const uint32_t grp = blockIdx.x / kBarGroup;
+    const uint32_t grp_size = min(kBarGroup, G - grp * kBarGroup);
+    for (;;) {
+        if (tid == 0) {
+            s_misc[2] = __hip_atomic_load(&ctrl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) |
+                        __hip_atomic_load(&ctrl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+        if (s_misc[2]) break;
+        const uint32_t wave = ctrl->wave;
+        const uint32_t cur = ctrl->cur;
+        const uint32_t nj = ctrl->n_jobs;
+        const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t p = wave & 1u;
+        for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
+        __syncthreads();
+        for (uint32_t j = blockIdx.x; j < nj; j += G) {
+            const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+            if (tid == 0) claim_tile(a, g, p);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32);
+            __syncthreads();
+        }
+        // ---- arrive ----
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t last = 0;
+            const uint32_t o = atomicAdd(&ctrl->arrive[grp * kCounterStride], 1u);
+            if (o == grp_size - 1u) {
+                const uint32_t t = atomicAdd(&ctrl->arrive_top, 1u);
+                last = (t == n_groups - 1u) ? 1u : 0u;
+            }
+            s_misc[3] = last;
+        }
+        __syncthreads();
+        if (s_misc[3]) {
+            __threadfence();
+            wave_select<kSweepThreads, 0>(a, false, hist, part, s_prefix, s_misc, nullptr, nullptr);
+            if (tid < (int)n_groups) ctrl->arrive[tid * kCounterStride] = 0u;
+            if (tid == 0) ctrl->arrive_top = 0u;
+            __threadfence();
+            __syncthreads();
+            if (tid == 0)
+                __hip_atomic_store(&ctrl->wave_ready, wave + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (tid == 0) {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(&ctrl->wave_ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != wave + 1u) {
+                if (__hip_atomic_load(&ctrl->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (++polls > (1u << 26)) {
+                    atomicOr(&ctrl->abort, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 }
 
@@ -592,32 +1054,120 @@ static hipError_t launch_solve_t(const SolveArgs& a, uint32_t grid, hipStream_t 
     return hipGetLastError();
 }
 
-template <int TW, int TH, int MODE, bool HOST>
-static solve_launcher_t pick_kind(int kind) {
-    switch (kind) {
-        case 0: return &launch_solve_t<TW, TH, 0, MODE, HOST>;
-        case 1: return &launch_solve_t<TW, TH, 1, MODE, HOST>;
-        case 2: return &launch_solve_t<TW, TH, 2, MODE, HOST>;
-        case 3: return &launch_solve_t<TW, TH, 3, MODE, HOST>;
-    }
-    return nullptr;
+template <int TW, int TH, int KIND, int MODE, bool HOST>
+static int occ_solve_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve<TW, TH, KIND, MODE, HOST>, 64, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <int TW, int TH, int KIND, bool HOST>
+static int occ_sweep_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep<TW, TH, KIND, HOST>,
+                                                     kSweepThreads, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <int TW, int TH, int KIND, bool HOST>
+static hipError_t launch_sweep_t(const SolveArgs& a, uint32_t grid, hipStream_t s,
+                                 hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST>), dim3(grid), dim3(kSweepThreads),
+                              0, s, ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST>), dim3(grid), dim3(kSweepThreads), 0,
+                           s, a);
+    return hipGetLastError();
+}
+
+template <int TW, int TH, int KIND>
+static hipError_t launch_persistent_t(const SolveArgs& a, uint32_t grid, hipStream_t s) {
+    void* args[] = {const_cast<SolveArgs*>(&a)};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_persistent_sweep<TW, TH, KIND>),
+                                      dim3(grid), dim3(kSweepThreads), args, 0, s);
+}
+
+template <int TW, int TH, int KIND>
+static int occ_persistent_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_persistent_sweep<TW, TH, KIND>,
+                                                     kSweepThreads, 0) != hipSuccess)
+        return 0;
+    return nb;
 }
 
 template <int TW, int TH>
-static solve_launcher_t pick_mode(int kind, int mode, bool host) {
+static PersistentKernel pick_persistent(int kind) {
+    switch (kind) {
+        case 0: return {&launch_persistent_t<TW, TH, 0>, &occ_persistent_t<TW, TH, 0>};
+        case 1: return {&launch_persistent_t<TW, TH, 1>, &occ_persistent_t<TW, TH, 1>};
+        case 2: return {&launch_persistent_t<TW, TH, 2>, &occ_persistent_t<TW, TH, 2>};
+        case 3: return {&launch_persistent_t<TW, TH, 3>, &occ_persistent_t<TW, TH, 3>};
+    }
+    return {nullptr, nullptr};
+}
+
+PersistentKernel get_persistent_kernel(int tw, int th, int kind, int mode) {
+    if (mode != 2) return {nullptr, nullptr};
+    if (tw == 64 && th == 64) return pick_persistent<64, 64>(kind);
+    if (tw == 64 && th == 32) return pick_persistent<64, 32>(kind);
+    if (tw == 64 && th == 16) return pick_persistent<64, 16>(kind);
+    if (tw == 32 && th == 32) return pick_persistent<32, 32>(kind);
+    return {nullptr, nullptr};
+}
+
+template <int TW, int TH, bool HOST>
+static SolveKernel pick_sweep_kind(int kind) {
+    switch (kind) {
+        case 0: return {&launch_sweep_t<TW, TH, 0, HOST>, &occ_sweep_t<TW, TH, 0, HOST>};
+        case 1: return {&launch_sweep_t<TW, TH, 1, HOST>, &occ_sweep_t<TW, TH, 1, HOST>};
+        case 2: return {&launch_sweep_t<TW, TH, 2, HOST>, &occ_sweep_t<TW, TH, 2, HOST>};
+        case 3: return {&launch_sweep_t<TW, TH, 3, HOST>, &occ_sweep_t<TW, TH, 3, HOST>};
+    }
+    return {nullptr, nullptr};
+}
+
+template <int TW, int TH>
+static SolveKernel pick_sweep(int kind, bool host) {
+    return host ? pick_sweep_kind<TW, TH, true>(kind) : pick_sweep_kind<TW, TH, false>(kind);
+}
+
+template <int TW, int TH, int MODE, bool HOST>
+static SolveKernel pick_kind(int kind) {
+    switch (kind) {
+        case 0: return {&launch_solve_t<TW, TH, 0, MODE, HOST>, &occ_solve_t<TW, TH, 0, MODE, HOST>};
+        case 1: return {&launch_solve_t<TW, TH, 1, MODE, HOST>, &occ_solve_t<TW, TH, 1, MODE, HOST>};
+        case 2: return {&launch_solve_t<TW, TH, 2, MODE, HOST>, &occ_solve_t<TW, TH, 2, MODE, HOST>};
+        case 3: return {&launch_solve_t<TW, TH, 3, MODE, HOST>, &occ_solve_t<TW, TH, 3, MODE, HOST>};
+    }
+    return {nullptr, nullptr};
+}
+
+template <int TW, int TH>
+static SolveKernel pick_mode(int kind, int mode, bool host) {
     if (mode == MODE_SYSTOLIC) return host ? pick_kind<TW, TH, MODE_SYSTOLIC, true>(kind)
                                            : pick_kind<TW, TH, MODE_SYSTOLIC, false>(kind);
     if (mode == MODE_FIM) return host ? pick_kind<TW, TH, MODE_FIM, true>(kind)
                                       : pick_kind<TW, TH, MODE_FIM, false>(kind);
-    return nullptr;
+    return {nullptr, nullptr};
 }
 
-solve_launcher_t get_solve_launcher(int tw, int th, int kind, int mode, bool host) {
+SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host) {
+    if (mode == 2) {
+        if (tw == 64 && th == 64) return pick_sweep<64, 64>(kind, host);
+        if (tw == 64 && th == 32) return pick_sweep<64, 32>(kind, host);
+        if (tw == 64 && th == 16) return pick_sweep<64, 16>(kind, host);
+        if (tw == 32 && th == 32) return pick_sweep<32, 32>(kind, host);
+        return {nullptr, nullptr};
+    }
     if (tw == 8 && th == 8) return pick_mode<8, 8>(kind, mode, host);
     if (tw == 16 && th == 16) return pick_mode<16, 16>(kind, mode, host);
     if (tw == 32 && th == 8) return pick_mode<32, 8>(kind, mode, host);
     if (tw == 32 && th == 32) return pick_mode<32, 32>(kind, mode, host);
-    return nullptr;
+    return {nullptr, nullptr};
 }
 
 hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s) {

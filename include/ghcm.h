@@ -43,7 +43,10 @@ typedef enum ghcm_status {
 /* In-tile solver style (HcmConfig::Kernel, src/eikonal.h:13). */
 typedef enum ghcm_kernel {
     GHCM_KERNEL_SYSTOLIC = 0,      /* data/eikonal_ghcm_systolic.cl */
-    GHCM_KERNEL_FIM      = 1       /* data/eikonal_ghcm_fim.cl      */
+    GHCM_KERNEL_FIM      = 1,      /* data/eikonal_ghcm_fim.cl      */
+    GHCM_KERNEL_SWEEP    = 2       /* systolic relaxation by four concurrent fast-sweeping
+                                      orders, one wave64 each; tiles 64x64, 64x32, 64x16,
+                                      32x32 */
 } ghcm_kernel;
 
 /* Who picks the n lowest-min-time tiles of a wave. */
@@ -81,10 +84,14 @@ typedef struct ghcm_config {
     int32_t  scheduler;             /* ghcm_scheduler                                 */
     uint32_t inner_iter_cap;        /* in-tile passes per visit, 0 = until converged  */
     uint32_t max_waves;             /* safety cap on waves, 0 = automatic             */
-    uint32_t waves_per_poll;        /* device scheduler: waves enqueued between host
-                                       polls of the done flag, 0 = automatic          */
+    uint32_t waves_per_poll;        /* device scheduler: waves per enqueued chunk (4
+                                       chunks in flight between polls), 0 = 4         */
     uint32_t profile_kernels;       /* 1: time every tile-solve launch with HIP events
                                        (fills solve_kernel_us / solve_launches)       */
+    uint32_t launch_mode;           /* device scheduler: 0 = one tile-solve launch per
+                                       wave (default), 1 = one persistent cooperative
+                                       launch runs every wave where built (sweep kernel;
+                                       its grid barrier pays agent-scope fences)      */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */

@@ -48,7 +48,7 @@ def test_config_defaults_mirror_hcmconfig():
 
 
 def test_struct_sizes_match_header():
-    assert ctypes.sizeof(g.Config) == 52
+    assert ctypes.sizeof(g.Config) == 56
     assert ctypes.sizeof(g.Stats) == 104
     assert ctypes.sizeof(g.Instance) == 40
 

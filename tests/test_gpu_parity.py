@@ -19,6 +19,7 @@ TOL = 1e-4          # max |dT| <= TOL * max T (north-star tolerance for the FP32
 
 TILES = [(8, 8), (16, 16), (32, 8), (32, 32)]
 KERNELS = [g.KERNEL_SYSTOLIC, g.KERNEL_FIM]
+SWEEP_TILES = [(64, 64), (64, 32), (64, 16), (32, 32)]
 
 
 def check(t, ref, seeds, cost=None, tol=TOL, label="", eps=1e-5):
@@ -58,6 +59,21 @@ def test_c1_rings_every_config(c1, kernel, tile, sched):
     check(t, ref64.astype(np.float64), seeds, label="vs fmm64 fixture")
 
 
+@pytest.mark.parametrize("tile", SWEEP_TILES)
+@pytest.mark.parametrize("sched,launch_mode", [(g.SCHED_DEVICE_TOPN, 0), (g.SCHED_DEVICE_TOPN, 1),
+                                               (g.SCHED_HOST_PQ, 0)])
+@pytest.mark.parametrize("cap", [0, 1])
+def test_c1_rings_sweep_kernel(c1, tile, sched, launch_mode, cap):
+    cost, seeds, ref32, ref64 = c1
+    print(f"case c1 sweep tile={tile} sched={sched} launch_mode={launch_mode} cap={cap}", flush=True)
+    with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=g.KERNEL_SWEEP, scheduler=sched,
+                   launch_mode=launch_mode, inner_iter_cap=cap, n_active=64) as ctx:
+        t, st = ctx.solve(cost, seeds)
+    assert st["converged"] == 1
+    check(t, ref32, seeds, cost, label="sweep vs fmm32")
+    check(t, ref64.astype(np.float64), seeds, label="sweep vs fmm64 fixture")
+
+
 @pytest.mark.parametrize("n_active", [1, 3, 64, 100000])
 def test_top_n_sizes(c1, n_active):
     """n smaller than the active set exercises the device top-n selection."""
@@ -70,7 +86,7 @@ def test_top_n_sizes(c1, n_active):
 SHAPES = [(1, 1), (1, 37), (37, 1), (13, 29), (100, 70), (3, 250), (257, 129)]
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS + [g.KERNEL_SWEEP])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_ragged_obstacles_multi_seed(rng, kernel, shape):
     h, w = shape
@@ -81,11 +97,14 @@ def test_ragged_obstacles_multi_seed(rng, kernel, shape):
     for x, y in seeds:
         cost[y, x] = 1.0
     ref = oracle.fmm_f32(cost, seeds)
-    for tile in ((8, 8), (32, 8)):
-        print(f"case ragged {shape} kernel={kernel} tile={tile}", flush=True)
-        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=7) as ctx:
-            t, _ = ctx.solve(cost, seeds)
-        check(t, ref, seeds, cost, label=f"{shape} {tile}")
+    tiles = ((64, 16), (32, 32)) if kernel == g.KERNEL_SWEEP else ((8, 8), (32, 8))
+    for tile in tiles:
+        for lm in ((0, 1) if kernel == g.KERNEL_SWEEP else (0,)):
+            print(f"case ragged {shape} kernel={kernel} tile={tile} launch_mode={lm}", flush=True)
+            with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=7,
+                           launch_mode=lm) as ctx:
+                t, _ = ctx.solve(cost, seeds)
+            check(t, ref, seeds, cost, label=f"{shape} {tile} lm={lm}")
 
 
 def test_adjacent_seeds_and_edges():
@@ -125,10 +144,11 @@ def test_strip_prefix_sums():
 
 def test_epsilon_zero_exact_convergence(c1):
     cost, seeds, ref32, _ = c1
-    for kernel in KERNELS:
-        with g.Context(0, min_change=0.0, kernel=kernel) as ctx:
+    for kernel in KERNELS + [g.KERNEL_SWEEP]:
+        tile = (64, 64) if kernel == g.KERNEL_SWEEP else (8, 8)
+        with g.Context(0, min_change=0.0, kernel=kernel, tile_w=tile[0], tile_h=tile[1]) as ctx:
             t, _ = ctx.solve(cost, seeds)
-        check(t, ref32, seeds, cost, tol=1e-5, label=f"eps=0 kernel={kernel}")
+        check(t, ref32, seeds, cost, tol=1e-5, label=f"eps=0 kernel={kernel}", eps=0.0)
 
 
 def test_metamorphic_configs_agree(rng):
@@ -140,6 +160,10 @@ def test_metamorphic_configs_agree(rng):
         for kernel in KERNELS:
             with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=50) as ctx:
                 outs.append(ctx.solve(cost, seeds)[0])
+    for tile in SWEEP_TILES:
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=g.KERNEL_SWEEP, n_active=50,
+                       inner_iter_cap=1) as ctx:
+            outs.append(ctx.solve(cost, seeds)[0])
     ref = oracle.fmm_f32(cost, seeds)
     for t in outs:
         check(t, ref, seeds, cost)

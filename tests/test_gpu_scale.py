@@ -20,9 +20,11 @@ def c2():
 def test_c2_full_size_parity(c2):
     speed, cost, seeds = c2
     ref = oracle.fmm_f32(cost, seeds)
-    for tile, kernel in (((16, 16), g.KERNEL_SYSTOLIC), ((8, 8), g.KERNEL_FIM)):
-        print(f"case c2 tile={tile} kernel={kernel}", flush=True)
-        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=16384) as ctx:
+    for tile, kernel, lm in (((16, 16), g.KERNEL_SYSTOLIC, 0), ((8, 8), g.KERNEL_FIM, 0),
+                             ((64, 64), g.KERNEL_SWEEP, 0), ((64, 64), g.KERNEL_SWEEP, 1)):
+        print(f"case c2 tile={tile} kernel={kernel} launch_mode={lm}", flush=True)
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=512,
+                       inner_iter_cap=1 if kernel == g.KERNEL_SWEEP else 0, launch_mode=lm) as ctx:
             dc = ctx.to_device(cost)
             dt = ctx.alloc(cost.shape, np.float32)
             st = ctx.solve_device(dc.ptr, g.FIELD_COST_F32, 4096, 4096, seeds, dt.ptr)
@@ -51,12 +53,13 @@ def test_precision_storage_vs_fp64(kind, tol):
         field = speed.astype(np.float16)
     else:
         field = g.to_bf16(speed.astype(np.float32))
-    with g.Context(0, tile_w=16, tile_h=16) as ctx:
-        t, _ = ctx.solve(field, seeds, kind=k)
-    m = oracle.error_metrics(t, ref64)
-    print(f"{kind}: linf={m['linf']:.3e} l1={m['l1']:.3e} max={m['max_ref']:.1f}")
-    assert m["finiteness_mismatch"] == 0
-    assert m["linf"] <= tol * m["max_ref"]
+    for kernel, tile in ((g.KERNEL_SYSTOLIC, (16, 16)), (g.KERNEL_SWEEP, (64, 64))):
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel) as ctx:
+            t, _ = ctx.solve(field, seeds, kind=k)
+        m = oracle.error_metrics(t, ref64)
+        print(f"{kind} kernel={kernel}: linf={m['linf']:.3e} l1={m['l1']:.3e} max={m['max_ref']:.1f}")
+        assert m["finiteness_mismatch"] == 0
+        assert m["linf"] <= tol * m["max_ref"]
 
 
 def test_ring_contrast_1e4_f16():
@@ -84,7 +87,12 @@ def test_batch_equals_one_at_a_time():
         seeds = g.gen_points(w, h, 200 + i, 1 + i % 4)
         cost = g.cost_from_speed(speed)
         insts.append((cost, seeds))
-    with g.Context(0, tile_w=16, tile_h=16, n_active=4096) as ctx:
+    for kernel, tile in ((g.KERNEL_SYSTOLIC, (16, 16)), (g.KERNEL_SWEEP, (64, 64))):
+        _batch_case(insts, kernel, tile)
+
+
+def _batch_case(insts, kernel, tile):
+    with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=4096) as ctx:
         dev = [(ctx.to_device(c), ctx.alloc(c.shape, np.float32)) for c, _ in insts]
         st = ctx.solve_batch(g.FIELD_COST_F32, [(dc.ptr, dt.ptr, c.shape[1], c.shape[0], s)
                                                 for (dc, dt), (c, s) in zip(dev, insts)])
