@@ -77,6 +77,148 @@ def cpu_baseline(cost, seeds, trials: int):
                            f"{med:.2f} s, pinned to one core"}
 
 
+BATCH_CANDIDATES = [  # autotuner search space for the batch (kept small so the bench stays short)
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=16384, inner_iter_cap=1, launch_mode=0),
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=0),
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=2, launch_mode=2),
+    dict(tile_w=64, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
+    dict(tile_w=32, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
+    dict(tile_w=16, tile_h=16, kernel=0, n_active=65536, inner_iter_cap=0, launch_mode=2),
+]
+
+
+def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
+    """512 uneven instances in one device wave loop vs the same solves one at a time."""
+    import numpy as np
+    from ghcm_amd import workloads
+    t0 = time.perf_counter()
+    insts = workloads.batch()
+    log(f"bench: batch of {len(insts)} generated in {time.perf_counter() - t0:.1f} s")
+    cells = sum(i.w * i.h for i in insts)
+    with g.Context(0, **plain_cfg) as ctx:
+        dev = [(ctx.to_device(i.cost), ctx.alloc((i.h, i.w), np.float32)) for i in insts]
+        args = [(dc.ptr, dt.ptr, i.w, i.h, i.seeds) for (dc, dt), i in zip(dev, insts)]
+        for a in args[:8]:                                   # warm-up
+            ctx.solve_device(a[0], g.FIELD_COST_F32, a[2], a[3], a[4], a[1])
+        t0 = time.perf_counter()
+        for a in args:
+            ctx.solve_device(a[0], g.FIELD_COST_F32, a[2], a[3], a[4], a[1])
+        t_plain = time.perf_counter() - t0
+        ref = [dt.download() for _, dt in dev]
+        best, res = ctx.autotune(g.FIELD_COST_F32, args, BATCH_CANDIDATES, reps=1)
+        log(f"bench: batch autotune -> {best}")
+        ctx.solve_batch(g.FIELD_COST_F32, args)
+        ts = []
+        for _ in range(max(1, steps)):
+            t0 = time.perf_counter()
+            st = ctx.solve_batch(g.FIELD_COST_F32, args)
+            ts.append(time.perf_counter() - t0)
+        t_batch = statistics.median(ts)
+        worst = 0.0
+        for (_, dt), r in zip(dev, ref):
+            t = dt.download()
+            fin = np.isfinite(r)
+            if not np.array_equal(fin, np.isfinite(t)):
+                worst = float("inf")
+                break
+            if fin.any():
+                worst = max(worst, float(np.max(np.abs(t[fin] - r[fin])) / max(1e-30, r[fin].max())))
+        for dc, dt in dev:
+            dc.free()
+            dt.free()
+    keys = ("tile_w", "tile_h", "kernel", "n_active", "inner_iter_cap", "launch_mode")
+    return {"workload": f"{len(insts)} uneven instances, sides 64..2048, {cells / 1e6:.0f} Mcells, "
+                        "ring or smooth, 1-8 sources (seed 4040)",
+            "solves_per_s": round(len(insts) / t_batch, 1),
+            "plain_solves_per_s": round(len(insts) / t_plain, 1),
+            "speedup_vs_plain": round(t_plain / t_batch, 2),
+            "batch_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
+            "batch_mcells_per_s": round(cells / t_batch / 1e6, 1),
+            "waves": st["waves"], "converged": st["converged"],
+            "autotuned": {k: best[k] for k in keys},
+            "autotune_device_ms": {f"{c['tile_w']}x{c['tile_h']}/k{c['kernel']}/n{c['n_active']}/"
+                                   f"cap{c['inner_iter_cap']}/lm{c['launch_mode']}":
+                                   round(c["device_us"] / 1e3, 3) for c in res},
+            "max_rel_diff_vs_plain": worst}
+
+
+def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256) -> dict:
+    """Continuous batching: saturated throughput, then Poisson arrivals for p50/p99."""
+    import threading
+
+    import numpy as np
+    from ghcm_amd import workloads
+    _, insts = workloads.serving_requests(n_distinct, 1.0)
+    reqs = [insts[i % n_distinct] for i in range(n_requests)]
+    # plain: one ghcm_solve (host buffers in and out) after another
+    with g.Context(0, **serve_cfg) as ctx:
+        for r in insts[:4]:
+            ctx.solve(r.cost, r.seeds)
+        plain = {}
+        t0 = time.perf_counter()
+        for i, r in enumerate(insts):
+            plain[i], _ = ctx.solve(r.cost, r.seeds)
+        t_plain = (time.perf_counter() - t0) / n_distinct
+    out = {"workload": f"{n_requests} requests cycling {n_distinct} square fields, sides "
+                       "log-uniform 128..2048, ring or smooth, 1-4 sources (seed 5050), host "
+                       "buffers in and out",
+           "plain_solves_per_s": round(1.0 / t_plain, 1)}
+    with g.Server(0, **serve_cfg) as srv:
+        for r in insts[:4]:
+            srv.wait(srv.submit(r.cost, r.seeds))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(r.cost, r.seeds) for r in reqs]
+        res = [srv.wait(t) for t in tickets]
+        t_sat = time.perf_counter() - t0
+        worst = 0.0
+        for i, (t, info) in enumerate(res):
+            p = plain[i % n_distinct]
+            fin = np.isfinite(p)
+            if fin.any():
+                worst = max(worst, float(np.max(np.abs(t[fin] - p[fin])) / p[fin].max()))
+        sat_rate = n_requests / t_sat
+        # open loop at half the saturated rate
+        rate = 0.5 * sat_rate
+        gaps = np.random.default_rng(5051).exponential(1.0 / rate, n_requests)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        tickets = [None] * n_requests
+        submitted = threading.Event()
+
+        def client():
+            start = time.perf_counter()
+            for i, r in enumerate(reqs):
+                d = arrivals[i] - (time.perf_counter() - start)
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(r.cost, r.seeds)
+            submitted.set()
+
+        th = threading.Thread(target=client)
+        t0 = time.perf_counter()
+        th.start()
+        lat = []
+        for i in range(n_requests):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            _, info = srv.wait(tickets[i])
+            lat.append(info["latency_us"] / 1e3)
+        th.join()
+        t_open = time.perf_counter() - t0
+        sst = srv.stats()
+    lat = np.array(lat)
+    out.update({"saturated_solves_per_s": round(sat_rate, 1),
+                "speedup_vs_plain": round(sat_rate * t_plain, 2),
+                "poisson_rate_per_s": round(rate, 1),
+                "poisson_achieved_per_s": round(n_requests / t_open, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "max_ms": round(float(lat.max()), 3),
+                "peak_live": sst["peak_live"],
+                "max_rel_diff_vs_plain": worst})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +232,8 @@ def main() -> int:
     ap.add_argument("--launch-mode", type=int, default=0, help="0 per-wave launches, 1 persistent")
     ap.add_argument("--cpu-trials", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve line")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (rocprofv3 --pmc scaling)")
     args = ap.parse_args()
@@ -181,6 +325,22 @@ def main() -> int:
                                                      max(1, sts[0]["tile_visits"]), 2),
                      "device_ms": round(statistics.median(s["duration_device_us"] for s in sts) / 1e3, 4)},
            "roofline": roof}
+
+    plain_cfg = dict(tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
+                     inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,
+                     launch_mode=args.launch_mode)
+    if not args.no_batch:
+        log("bench: batched solves")
+        out["batched"] = bench_batch(g, plain_cfg, 3)
+        log(f"bench: batched {out['batched']['solves_per_s']} solves/s, "
+            f"{out['batched']['speedup_vs_plain']}x plain")
+    if not args.no_serve:
+        log("bench: serving")
+        scfg = dict(plain_cfg)
+        if "autotuned" in out.get("batched", {}):
+            scfg.update(out["batched"]["autotuned"])
+        out["serving"] = bench_serving(g, scfg)
+        log(f"bench: serving p50 {out['serving']['p50_ms']} ms p99 {out['serving']['p99_ms']} ms")
 
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline ({args.cpu_trials} trials of the oracle FMM)")

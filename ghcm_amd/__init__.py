@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "libghcm.so")
 
 # ghcm_status
 OK, ERR_INVALID_ARG, ERR_NO_DEVICE, ERR_HIP, ERR_OUT_OF_MEMORY, ERR_NOT_CONVERGED, \
-    ERR_UNSUPPORTED, ERR_OVER_BUDGET, ERR_NOT_FOUND, ERR_STOPPED = range(10)
+    ERR_UNSUPPORTED, ERR_OVER_BUDGET, ERR_NOT_FOUND, ERR_STOPPED, ERR_TIMEOUT = range(11)
 # ghcm_kernel / ghcm_scheduler / ghcm_field_kind
 KERNEL_SYSTOLIC, KERNEL_FIM, KERNEL_SWEEP = 0, 1, 2
 SCHED_DEVICE_TOPN, SCHED_HOST_PQ = 0, 1
@@ -72,11 +72,47 @@ class DeviceInfo(ctypes.Structure):
                 ("pci_bus_id", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class TuneCandidate(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_uint32), ("tile_h", ctypes.c_uint32),
+                ("kernel", ctypes.c_int32), ("n_active", ctypes.c_uint32),
+                ("inner_iter_cap", ctypes.c_uint32), ("launch_mode", ctypes.c_uint32),
+                ("device_us", ctypes.c_double), ("tile_visits", ctypes.c_uint64),
+                ("converged", ctypes.c_uint32), ("status", ctypes.c_int32)]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Instance(ctypes.Structure):
     _fields_ = [("d_field", ctypes.c_void_p), ("d_time", ctypes.c_void_p),
                 ("w", ctypes.c_uint32), ("h", ctypes.c_uint32),
                 ("seeds_xy", ctypes.c_void_p), ("n_seeds", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_tiles", ctypes.c_uint32),
+                ("max_slots", ctypes.c_uint32), ("field_kind", ctypes.c_int32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("w", ctypes.c_uint32), ("h", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("waves", ctypes.c_uint64),
+                ("queue_us", ctypes.c_double), ("latency_us", ctypes.c_double)]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in (
+        "submitted", "admitted", "completed", "pending", "waves_enqueued", "busy_waves",
+        "loop_wave", "peak_live", "budget_bytes", "bytes_in_use", "peak_bytes", "max_tiles")] + \
+        [(k, ctypes.c_uint32) for k in ("slots", "n_active", "waves_per_chunk", "reserved")]
+
+    def to_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 # Every symbol declared by include/ghcm.h and include/eikonal_c.h.
@@ -85,7 +121,10 @@ EXPORTED_SYMBOLS = (
     "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
     "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
     "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download", "ghcm_dev_copy",
-    "ghcm_solve_batch", "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
+    "ghcm_solve_batch", "ghcm_tune_default_space", "ghcm_autotune",
+    "ghcm_serve_config_default", "ghcm_serve_start", "ghcm_serve_submit", "ghcm_serve_wait",
+    "ghcm_serve_get_stats", "ghcm_serve_stop",
+    "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
     "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
 )
 
@@ -123,6 +162,15 @@ def lib() -> ctypes.CDLL:
     L.ghcm_solve.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_device.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_batch.argtypes = [vp, i32, P(Instance), u32, P(Stats)]
+    L.ghcm_tune_default_space.argtypes = [i32, P(TuneCandidate), u32, P(u32)]
+    L.ghcm_autotune.argtypes = [vp, i32, P(Instance), u32, P(TuneCandidate), u32, u32, P(u32)]
+    L.ghcm_serve_config_default.argtypes = [P(ServeConfig)]
+    L.ghcm_serve_config_default.restype = None
+    L.ghcm_serve_start.argtypes = [i32, P(Config), P(ServeConfig), P(vp)]
+    L.ghcm_serve_submit.argtypes = [vp, vp, u32, u32, vp, u32, vp, P(ctypes.c_uint64)]
+    L.ghcm_serve_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, P(ServeResult)]
+    L.ghcm_serve_get_stats.argtypes = [vp, P(ServeStats)]
+    L.ghcm_serve_stop.argtypes = [vp, i32]
     L.ghcm_dev_alloc.argtypes = [vp, ctypes.c_size_t, P(vp)]
     L.ghcm_dev_free.argtypes = [vp, vp]
     L.ghcm_dev_upload.argtypes = [vp, vp, vp, ctypes.c_size_t]
@@ -284,8 +332,8 @@ class Context:
                                          ctypes.byref(st)))
         return st.to_dict()
 
-    def solve_batch(self, kind: int, instances: Sequence[tuple]) -> dict:
-        """instances: (d_field_ptr, d_time_ptr, w, h, seeds) tuples, device pointers."""
+    @staticmethod
+    def _instances(instances):
         n = len(instances)
         arr = (Instance * n)()
         keep = []
@@ -293,9 +341,118 @@ class Context:
             s = _seeds_array(seeds)
             keep.append(s)
             arr[i] = Instance(df, dt, w, h, s.ctypes.data, s.shape[0], 0)
+        return arr, keep
+
+    def solve_batch(self, kind: int, instances: Sequence[tuple]) -> dict:
+        """instances: (d_field_ptr, d_time_ptr, w, h, seeds) tuples, device pointers."""
+        arr, keep = self._instances(instances)
         st = Stats()
-        _check(self._L.ghcm_solve_batch(self._h, kind, arr, n, ctypes.byref(st)))
+        _check(self._L.ghcm_solve_batch(self._h, kind, arr, len(instances), ctypes.byref(st)))
         return st.to_dict()
+
+    def autotune(self, kind: int, instances: Sequence[tuple], candidates=None, reps: int = 2):
+        """Time candidate configurations on the instances; the fastest becomes the config.
+
+        candidates: list of dicts (tile_w, tile_h, kernel, n_active, inner_iter_cap,
+        launch_mode) or None for the library's default space.  Returns (best, results).
+        """
+        if candidates is None:
+            cap = 1024
+            buf = (TuneCandidate * cap)()
+            n = ctypes.c_uint32(0)
+            cus = device_info(self.ordinal)["compute_units"]
+            _check(self._L.ghcm_tune_default_space(cus, buf, cap, ctypes.byref(n)))
+            cands = (TuneCandidate * n.value)(*buf[:n.value])
+        else:
+            cands = (TuneCandidate * len(candidates))()
+            for i, c in enumerate(candidates):
+                cands[i] = TuneCandidate(c["tile_w"], c["tile_h"], c["kernel"], c["n_active"],
+                                         c.get("inner_iter_cap", 0), c.get("launch_mode", 0),
+                                         0.0, 0, 0, 0)
+        arr, keep = self._instances(instances)
+        best = ctypes.c_uint32(0)
+        _check(self._L.ghcm_autotune(self._h, kind, arr, len(instances), cands, len(cands),
+                                     reps, ctypes.byref(best)))
+        c = Config()
+        _check(self._L.ghcm_ctx_get_config(self._h, ctypes.byref(c)))
+        self.config = c
+        return cands[best.value].to_dict(), [x.to_dict() for x in cands]
+
+
+class Server:
+    """Continuous-batching solve server (ghcm_serve_*): one device wave loop that admits
+    requests at wave boundaries while earlier ones are still being solved.
+
+    ``submit`` returns a ticket; the field array and the output array are kept alive by
+    this object until ``wait`` collects the ticket."""
+
+    def __init__(self, ordinal: int = 0, kind: int = FIELD_COST_F32, hbm_budget_bytes: int = 0,
+                 max_slots: int = 0, max_tiles: int = 0, **config):
+        self._L = lib()
+        self.kind = kind
+        self.config = default_config(**config)
+        sc = ServeConfig()
+        self._L.ghcm_serve_config_default(ctypes.byref(sc))
+        if hbm_budget_bytes:
+            sc.hbm_budget_bytes = hbm_budget_bytes
+        if max_slots:
+            sc.max_slots = max_slots
+        if max_tiles:
+            sc.max_tiles = max_tiles
+        sc.field_kind = kind
+        self.serve_config = sc
+        h = ctypes.c_void_p()
+        _check(self._L.ghcm_serve_start(ordinal, ctypes.byref(self.config), ctypes.byref(sc),
+                                        ctypes.byref(h)))
+        self._h = h
+        self._held = {}
+
+    def submit(self, field: np.ndarray, seeds) -> int:
+        f = _field_array(field, self.kind)
+        if f.ndim != 2:
+            raise ValueError("field must be 2-D (H, W)")
+        h, w = f.shape
+        s = _seeds_array(seeds)
+        out = np.empty((h, w), dtype=np.float32)
+        t = ctypes.c_uint64(0)
+        _check(self._L.ghcm_serve_submit(self._h, f.ctypes.data, w, h, s.ctypes.data, s.shape[0],
+                                         out.ctypes.data, ctypes.byref(t)))
+        self._held[t.value] = (f, out)
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = -1.0):
+        """(times (H, W) float32, result dict); raises GhcmError on failure or time-out."""
+        r = ServeResult()
+        rc = self._L.ghcm_serve_wait(self._h, ticket, timeout_s, ctypes.byref(r))
+        if rc == ERR_TIMEOUT:
+            _check(rc)
+        f, out = self._held.pop(ticket, (None, None))
+        _check(rc)
+        return out, r.to_dict()
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _check(self._L.ghcm_serve_get_stats(self._h, ctypes.byref(st)))
+        return st.to_dict()
+
+    def stop(self, drain: bool = True) -> None:
+        if getattr(self, "_h", None):
+            h, self._h = self._h, None
+            rc = self._L.ghcm_serve_stop(h, 1 if drain else 0)
+            self._held.clear()
+            _check(rc)
+
+    def __del__(self):
+        try:
+            self.stop(drain=False)
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop(drain=exc[0] is None)
 
 
 class DeviceArray:

@@ -1,0 +1,90 @@
+// capi_common.h -- helpers shared by the C-ABI translation units of libghcm: the
+// thread-local error message, HIP error propagation and grow-only device / pinned buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "ghcm.h"
+
+namespace ghcm_capi {
+
+// Records a printf-style message for ghcm_last_error() and returns `status`.
+int fail(int status, const char* fmt, ...);
+// Validation shared by contexts and servers.
+int check_config(const ghcm_config* c);
+
+inline bool valid_kind(int k) { return k >= GHCM_FIELD_COST_F32 && k <= GHCM_FIELD_SPEED_BF16; }
+inline size_t kind_bytes(int k) {
+    return (k == GHCM_FIELD_SPEED_F16 || k == GHCM_FIELD_SPEED_BF16) ? 2 : 4;
+}
+
+// Algorithmic HBM bytes of one tile visit: field read + time read (tile and the 4 halo
+// edges) + time write + 40 B of tile metadata.
+inline uint32_t bytes_per_visit(uint32_t tw, uint32_t th, int kind) {
+    const uint32_t P = tw * th;
+    return P * (uint32_t)kind_bytes(kind) + (P + 2 * (tw + th)) * 4 + P * 4 + 40;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return ::ghcm_capi::fail(GHCM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer that only grows; reused across solves.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want) {
+        if (want <= bytes) return GHCM_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(want, 256);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        }
+        bytes = b;
+        return GHCM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Pinned host buffer that only grows.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t want) {
+        if (want <= bytes) return GHCM_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t b = std::max<size_t>(want, 256);
+        hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        }
+        bytes = b;
+        return GHCM_OK;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace ghcm_capi

@@ -24,11 +24,20 @@
 
 using namespace ghcm;
 
+#include "capi_common.h"
+
+using ghcm_capi::DevBuf;
+using ghcm_capi::PinnedBuf;
+using ghcm_capi::bytes_per_visit;
+using ghcm_capi::fail;
+using ghcm_capi::kind_bytes;
+using ghcm_capi::valid_kind;
+
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int status, const char* fmt, ...) {
+int ghcm_capi::fail(int status, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -38,69 +47,6 @@ int fail(int status, const char* fmt, ...) {
     return status;
 }
 
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(GHCM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
-    } while (0)
-
-// Device buffer that only grows; reused across solves on one context.
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    int reserve(size_t want) {
-        if (want <= bytes) return GHCM_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t b = std::max<size_t>(want, 256);
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
-        }
-        bytes = b;
-        return GHCM_OK;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    int reserve(size_t want) {
-        if (want <= bytes) return GHCM_OK;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t b = std::max<size_t>(want, 256);
-        hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(GHCM_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", b, hipGetErrorString(e));
-        }
-        bytes = b;
-        return GHCM_OK;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-bool valid_kind(int k) { return k >= GHCM_FIELD_COST_F32 && k <= GHCM_FIELD_SPEED_BF16; }
-size_t kind_bytes(int k) { return (k == GHCM_FIELD_SPEED_F16 || k == GHCM_FIELD_SPEED_BF16) ? 2 : 4; }
-
-}  // namespace
-
 struct ghcm_ctx {
     int ordinal = 0;
     ghcm_config cfg{};
@@ -109,6 +55,10 @@ struct ghcm_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_prof;    // profile_kernels: begin/end pairs per queued wave
     std::vector<hipEvent_t> ev_slot;    // device scheduler: one per chunk in flight
+    // launch_mode 2: one chunk of waves captured as a HIP graph, reused while the solve
+    // arguments (buffers, sizes, n) and grid stay the same.
+    hipGraphExec_t gexec = nullptr;
+    std::vector<unsigned char> gkey;
     // device scratch
     DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals, sel;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
@@ -133,6 +83,7 @@ extern "C" const char* ghcm_status_string(int s) {
         case GHCM_ERR_OVER_BUDGET: return "request exceeds the HBM budget";
         case GHCM_ERR_NOT_FOUND: return "unknown ticket";
         case GHCM_ERR_STOPPED: return "server stopped";
+        case GHCM_ERR_TIMEOUT: return "wait timed out";
     }
     return "unknown status";
 }
@@ -196,7 +147,7 @@ extern "C" int ghcm_device_info_get(int ordinal, ghcm_device_info* out) {
 // context
 // ---------------------------------------------------------------------------------------
 
-static int check_config(const ghcm_config* c) {
+int ghcm_capi::check_config(const ghcm_config* c) {
     if (!c) return fail(GHCM_ERR_INVALID_ARG, "config is null");
     if (!(c->min_change >= 0.0f) || !std::isfinite(c->min_change))
         return fail(GHCM_ERR_INVALID_ARG, "min_change must be finite and >= 0");
@@ -221,7 +172,7 @@ extern "C" int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** o
     *out = nullptr;
     ghcm_config c;
     if (cfg) c = *cfg; else ghcm_config_default(&c);
-    int rc = check_config(&c);
+    int rc = ghcm_capi::check_config(&c);
     if (rc) return rc;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || ordinal < 0 || ordinal >= n)
@@ -263,6 +214,7 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : ctx->ev_slot)
         if (ev) (void)hipEventDestroy(ev);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return GHCM_OK;
@@ -276,7 +228,7 @@ extern "C" int ghcm_ctx_get_config(const ghcm_ctx* ctx, ghcm_config* out) {
 
 extern "C" int ghcm_ctx_set_config(ghcm_ctx* ctx, const ghcm_config* cfg) {
     if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
-    int rc = check_config(cfg);
+    int rc = ghcm_capi::check_config(cfg);
     if (rc) return rc;
     ctx->cfg = *cfg;
     return GHCM_OK;
@@ -296,11 +248,6 @@ struct HostInst {      // host view of one instance
     uint32_t n_seeds;
 };
 
-uint32_t bytes_per_visit(uint32_t tw, uint32_t th, int kind) {
-    // speed/cost read + time read (tile + 4 halo edges) + time write + 40 B metadata.
-    uint32_t P = tw * th;
-    return P * (uint32_t)kind_bytes(kind) + (P + 2 * (tw + th)) * 4 + P * 4 + 40;
-}
 
 int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats) {
     const auto t_wall0 = std::chrono::steady_clock::now();
@@ -489,16 +436,50 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             chunk_len[sl] = 0;
             return GHCM_OK;
         };
+        const bool use_graph = cfg.launch_mode == 2 && !prof;
+        if (use_graph) {
+            // key: the kernel arguments and launch shape of one chunk
+            std::vector<unsigned char> key(sizeof(SolveArgs) + 2 * sizeof(uint32_t));
+            std::memcpy(key.data(), &a, sizeof(SolveArgs));
+            std::memcpy(key.data() + sizeof(SolveArgs), &grid, sizeof(uint32_t));
+            std::memcpy(key.data() + sizeof(SolveArgs) + sizeof(uint32_t), &chunk, sizeof(uint32_t));
+            if (!ctx->gexec || key != ctx->gkey) {
+                if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+                ctx->gexec = nullptr;
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                hipError_t ce = hipSuccess;
+                for (uint32_t k = 0; k < chunk && ce == hipSuccess; ++k) {
+                    ce = launch(a, grid, st, nullptr, nullptr);
+                    if (ce == hipSuccess) ce = launch_wave_end(a, 0u, st);
+                }
+                hipError_t ee = hipStreamEndCapture(st, &graph);
+                if (ce != hipSuccess || ee != hipSuccess)
+                    return fail(GHCM_ERR_HIP, "graph capture failed: %s",
+                                hipGetErrorString(ce != hipSuccess ? ce : ee));
+                hipError_t ie = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    ctx->gexec = nullptr;
+                    return fail(GHCM_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+                }
+                ctx->gkey = key;
+            }
+        }
         uint64_t enqueued = 0;
         int inflight = 0, head = 0;                 // head: next slot to fill
         for (;;) {
-            for (uint32_t k = 0; k < chunk; ++k) {
-                if (prof)
-                    HIP_TRY(launch(a, grid, st, ctx->ev_prof[((size_t)head * chunk + k) * 2],
-                                   ctx->ev_prof[((size_t)head * chunk + k) * 2 + 1]));
-                else
-                    HIP_TRY(launch(a, grid, st, nullptr, nullptr));
-                HIP_TRY(launch_wave_end(a, 0u, st));
+            if (use_graph) {
+                HIP_TRY(hipGraphLaunch(ctx->gexec, st));
+            } else {
+                for (uint32_t k = 0; k < chunk; ++k) {
+                    if (prof)
+                        HIP_TRY(launch(a, grid, st, ctx->ev_prof[((size_t)head * chunk + k) * 2],
+                                       ctx->ev_prof[((size_t)head * chunk + k) * 2 + 1]));
+                    else
+                        HIP_TRY(launch(a, grid, st, nullptr, nullptr));
+                    HIP_TRY(launch_wave_end(a, 0u, st));
+                }
             }
             chunk_len[head] = prof ? chunk : 0;
             enqueued += chunk;

@@ -42,6 +42,7 @@ struct GhcmInst {
 constexpr uint32_t kShards = 32;        // list shards
 constexpr uint32_t kCounterStride = 32; // u32 between shard counters (128 B)
 
+constexpr uint32_t kMaxServeSlots = 1024; // instance slots of a serving loop
 constexpr uint32_t kBarGroup = 32;      // work-groups per arrival counter (persistent loop)
 constexpr uint32_t kMaxBarGroups = 64;
 
@@ -95,6 +96,8 @@ struct SolveArgs {
     uint32_t        shard_cap;
     uint32_t*       tstat;   // [3][tiles]
     uint32_t*       jobs;    // compacted top-n selection, capacity min(n, tiles)
+    uint32_t*       iact;    // serving: queued tiles per instance slot in list[cur]
+    uint32_t        n_iact;  // serving: instance slots (<= kMaxServeSlots)
     GhcmCtrl*       ctrl;
     float           eps;
     uint32_t        iter_cap;

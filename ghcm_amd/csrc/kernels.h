@@ -32,6 +32,10 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s);
 hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out,
                               hipStream_t s);
+hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
+                             uint32_t tile_w, uint32_t tile_h, hipStream_t s);
+hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
+                              hipStream_t s);
 hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
                        uint32_t tile_w, uint32_t tile_h, hipStream_t s);
 

@@ -893,7 +893,69 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     __shared__ uint32_t sg[kSelectStage];
     __shared__ uint32_t sk[kSelectStage];
     if (a.ctrl->done) return;
+    const uint32_t cur0 = a.ctrl->cur;
     wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
+    if (a.iact) {
+        // Serving: queued tiles per instance slot in the new current list (a slot whose
+        // count is 0 has nothing left to solve).  hist doubles as the counter array.
+        __syncthreads();
+        const int tid = threadIdx.x;
+        const uint32_t ns = a.n_iact;
+        for (uint32_t i = tid; i < ns; i += 1024) hist[i] = 0u;
+        __syncthreads();
+        const uint32_t c = s_prefix[kShards];
+        const uint32_t cur = first ? cur0 : (cur0 ^ 1u);
+        for (uint32_t i = tid; i < c; i += 1024)
+            atomicAdd(&hist[a.tinst[list_entry(a, s_prefix, cur, i)]], 1u);
+        __syncthreads();
+        for (uint32_t i = tid; i < ns; i += 1024) a.iact[i] = hist[i];
+    }
+}
+
+// Serving admission: seeds of newly admitted instances, (x, y, slot) triples.  Queues the
+// tiles of each seed and its 4 neighbours for the wave after the current one (they join
+// list[cur^1] like any activation) and clears `done` so an idle loop resumes.
+__global__ void k_seed_serve(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
+                             uint32_t tw, uint32_t th) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seeds) return;
+    const uint32_t wave = a.ctrl->wave;
+    const uint32_t cur = a.ctrl->cur;
+    const uint32_t np = (wave + 1u) & 1u;
+    const uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], ii = seeds[3 * s + 2];
+    const GhcmInst in = a.inst[ii];
+    in.time[(size_t)y * in.w + x] = 0.0f;
+    const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
+    for (int d = 0; d < 5; ++d) {
+        const int px = (int)x + dx[d], py = (int)y + dy[d];
+        if (px < 0 || py < 0 || px >= (int)in.w || py >= (int)in.h) continue;
+        const uint32_t t = in.tile_base + (uint32_t)(py / (int)th) * in.tiles_x + (uint32_t)(px / (int)tw);
+        atomicMin(&a.key[(size_t)np * a.tiles + t], 0u);
+        atomicOr(&a.mask[(size_t)np * a.tiles + t], kMaskOpenAll);
+        const uint32_t old = atomicMax(&a.qwave[t], wave + 2u);
+        if (old < wave + 2u) list_append(a, cur ^ 1u, t);
+    }
+    if (s == 0) a.ctrl->done = 0u;
+}
+
+// Serving admission: the slot's instance entry written, its tile range reset and mapped
+// to the slot.
+__global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<GhcmInst*>(a.inst)[slot] = in;
+    const uint32_t base = in.tile_base, count = in.tiles_x * in.tiles_y;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t t = base + i;
+        a.qwave[t] = 0u;
+        a.key[t] = kInfBits;
+        a.key[(size_t)a.tiles + t] = kInfBits;
+        a.mask[t] = 0u;
+        a.mask[(size_t)a.tiles + t] = 0u;
+        a.tmin[t] = __builtin_inff();
+        a.tstat[t] = 0u;
+        a.tstat[(size_t)a.tiles + t] = 0u;
+        a.tstat[2 * (size_t)a.tiles + t] = 0u;
+        const_cast<uint32_t*>(a.tinst)[t] = slot;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1195,6 +1257,23 @@ hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned l
     if (blocks > 1024) blocks = 1024;
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_reduce_stats, dim3(blocks), dim3(256), 0, s, tstat, tiles, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds, uint32_t n_seeds,
+                             uint32_t tw, uint32_t th, hipStream_t s) {
+    if (n_seeds == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seed_serve, dim3((n_seeds + 63) / 64), dim3(64), 0, s, a, seeds, n_seeds, tw, th);
+    return hipGetLastError();
+}
+
+hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
+                              hipStream_t s) {
+    const uint32_t count = in.tiles_x * in.tiles_y;
+    unsigned blocks = (count + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_admit_tiles, dim3(blocks), dim3(256), 0, s, a, in, slot);
     return hipGetLastError();
 }
 

@@ -37,7 +37,8 @@ typedef enum ghcm_status {
     GHCM_ERR_UNSUPPORTED     = 6,  /* valid enum, but no kernel built for that combination */
     GHCM_ERR_OVER_BUDGET     = 7,  /* serving: request larger than the HBM budget         */
     GHCM_ERR_NOT_FOUND       = 8,  /* serving: unknown ticket                             */
-    GHCM_ERR_STOPPED         = 9   /* serving: server stopped                             */
+    GHCM_ERR_STOPPED         = 9,  /* serving: server stopped                             */
+    GHCM_ERR_TIMEOUT         = 10  /* serving: wait timed out, the request is still running */
 } ghcm_status;
 
 /* In-tile solver style (HcmConfig::Kernel, src/eikonal.h:13). */
@@ -91,7 +92,8 @@ typedef struct ghcm_config {
     uint32_t launch_mode;           /* device scheduler: 0 = one tile-solve launch per
                                        wave (default), 1 = one persistent cooperative
                                        launch runs every wave where built (sweep kernel;
-                                       its grid barrier pays agent-scope fences)      */
+                                       its grid barrier pays agent-scope fences),
+                                       2 = chunks of waves replayed as a HIP graph    */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */
@@ -179,9 +181,103 @@ typedef struct ghcm_instance {
 int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst,
                      uint32_t n_inst, ghcm_stats* stats);
 
+/* ---- autotuner --------------------------------------------------------------
+ * Picks the tile shape/kernel, n, in-tile round cap and launch mode for a workload by
+ * timing candidate configurations on it (device time, best of `reps`).  The reference
+ * sweeps the same kind of grid by hand (src/test_runner.cpp:220-285). */
+typedef struct ghcm_tune_candidate {
+    uint32_t tile_w, tile_h;
+    int32_t  kernel;                /* ghcm_kernel                                    */
+    uint32_t n_active;
+    uint32_t inner_iter_cap;
+    uint32_t launch_mode;
+    double   device_us;             /* out: best device time (INFINITY if it failed)  */
+    uint64_t tile_visits;           /* out                                            */
+    uint32_t converged;             /* out                                            */
+    int32_t  status;                /* out: ghcm_status of the last run               */
+} ghcm_tune_candidate;
+
+/* Fills `out` (capacity `cap`) with the default search space for `compute_units` CUs:
+ * systolic and FIM tiles 8x8/16x16/32x8, sweep tiles 64x64/64x32/32x32, n in
+ * {1,2,4,16,64} x CUs, round caps and launch modes.  *n_out = number written. */
+int ghcm_tune_default_space(int compute_units, ghcm_tune_candidate* out, uint32_t cap,
+                            uint32_t* n_out);
+
+/* Times every candidate on the instances (device pointers, as ghcm_solve_batch) and
+ * leaves the fastest converged one as the context configuration (*best = its index). */
+int ghcm_autotune(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst, uint32_t n_inst,
+                  ghcm_tune_candidate* cands, uint32_t n_cands, uint32_t reps, uint32_t* best);
+
+/* ---- serving: continuous batching -------------------------------------------------
+ * No counterpart in the reference, which solves one field per call
+ * (src/eikonal_hcm.cpp:186-529); this is the serving form of the same wave loop.
+ * A server runs one device wave loop that never restarts between requests.  A request
+ * is admitted at a wave boundary when an instance slot, its tile state and its field +
+ * time buffer inside the HBM arena (`hbm_budget_bytes`) are free, first come first
+ * served; its tiles then join the same top-n selection as every live request's.  It
+ * completes at the first wave boundary at which none of its tiles is queued, and its
+ * times are copied into `time_out`.
+ * Threading: submit / wait / get_stats may be called from any threads; stop must not
+ * race with them. */
+typedef struct ghcm_serve_config {
+    uint64_t hbm_budget_bytes;      /* arena for request fields and times, 0 = 8 GiB  */
+    uint32_t max_tiles;             /* tile-state capacity, 0 = from the budget       */
+    uint32_t max_slots;             /* requests on the device at once, 0 = 256, <= 1024 */
+    int32_t  field_kind;            /* ghcm_field_kind of every request               */
+    uint32_t reserved[3];
+} ghcm_serve_config;
+
+typedef struct ghcm_serve_result {
+    int32_t  status;                /* ghcm_status of the request                     */
+    uint32_t w, h;
+    uint32_t reserved;
+    uint64_t waves;                 /* loop waves from admission to completion
+                                       (counted in whole chunks of waves_per_poll)    */
+    double   queue_us;              /* submit -> admission                            */
+    double   latency_us;            /* submit -> times written to time_out            */
+} ghcm_serve_result;
+
+typedef struct ghcm_serve_stats {
+    uint64_t submitted, admitted, completed, pending;
+    uint64_t waves_enqueued;        /* loop waves launched                            */
+    uint64_t busy_waves;            /* waves that had at least one queued tile        */
+    uint64_t loop_wave;             /* the loop's wave counter at the last poll       */
+    uint64_t peak_live;             /* most requests on the device at once            */
+    uint64_t budget_bytes, bytes_in_use, peak_bytes;
+    uint64_t max_tiles;
+    uint32_t slots, n_active, waves_per_chunk, reserved;
+} ghcm_serve_stats;
+
+typedef struct ghcm_server ghcm_server;
+
+void ghcm_serve_config_default(ghcm_serve_config* scfg);
+/* cfg: solver configuration (device top-n scheduler; launch_mode 2 replays each chunk
+ * of waves as one HIP graph, 1 behaves as 0).  Allocates the arena and tile state. */
+int ghcm_serve_start(int ordinal, const ghcm_config* cfg, const ghcm_serve_config* scfg,
+                     ghcm_server** out);
+/* Queues one solve.  `field` (W*H elements of the server's field kind) is read when the
+ * request is admitted and must stay valid until it completes; the seeds are copied
+ * here; `time_out` (W*H floats, host memory) receives the times.  GHCM_ERR_OVER_BUDGET
+ * if the request can never fit. */
+int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, uint32_t h,
+                      const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
+                      uint64_t* ticket);
+/* Waits up to timeout_s seconds (< 0: no limit).  Returns the request's status (and
+ * collects the ticket) or GHCM_ERR_TIMEOUT (the ticket stays valid). */
+int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s, ghcm_serve_result* out);
+int ghcm_serve_get_stats(ghcm_server* s, ghcm_serve_stats* out);
+/* drain != 0: every submitted request completes first; 0: requests not yet complete
+ * end with GHCM_ERR_STOPPED.  Frees the server. */
+int ghcm_serve_stop(ghcm_server* s, int drain);
+
 /* ---- synthetic fields (seeded, documented generators) ---------------------- */
-/* splitmix64-based; see DESIGN.md for the exact definitions.  Output is FP64
- * speed (H x W, row-major) so every storage precision derives from one field. */
+/* splitmix64-based, defined in ghcm_amd/csrc/field_gen.cpp.  Output is FP64 speed
+ * (H x W, row-major) so every storage precision derives from one field.
+ * ring:   concentric ring walls of speed `wall_speed` (one gap each) on speed 1, the
+ *         256^2 geometry scaled by `scale`;
+ * smooth: standard-normal noise smoothed by a Gaussian of `sigma` cells (reflecting
+ *         boundary), mapped log-linearly onto [speed_lo, speed_hi];
+ * points: n uniform cells.  */
 int ghcm_gen_ring_speed(uint32_t w, uint32_t h, uint64_t seed, double wall_speed,
                         double scale, double* speed_out);
 int ghcm_gen_smooth_speed(uint32_t w, uint32_t h, uint64_t seed, double sigma,

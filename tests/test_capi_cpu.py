@@ -51,6 +51,38 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(g.Config) == 56
     assert ctypes.sizeof(g.Stats) == 104
     assert ctypes.sizeof(g.Instance) == 40
+    assert ctypes.sizeof(g.ServeConfig) == 32
+    assert ctypes.sizeof(g.ServeResult) == 40
+    assert ctypes.sizeof(g.ServeStats) == 112
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler gives ghcm.h."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"ghcm_config": g.Config, "ghcm_stats": g.Stats, "ghcm_instance": g.Instance,
+               "ghcm_tune_candidate": g.TuneCandidate, "ghcm_device_info": g.DeviceInfo,
+               "ghcm_serve_config": g.ServeConfig, "ghcm_serve_result": g.ServeResult,
+               "ghcm_serve_stats": g.ServeStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghcm.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, got):
+        cname, what, val = line.split()
+        cls = mirrors[cname]
+        want = ctypes.sizeof(cls) if what == "size" else getattr(cls, what).offset
+        assert int(val) == want, (cname, what)
 
 
 @pytest.mark.skipif(g.device_count() > 0, reason="checks the no-device error path")

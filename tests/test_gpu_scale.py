@@ -105,3 +105,53 @@ def _batch_case(insts, kernel, tile):
             assert np.array_equal(fin, np.isfinite(tb))
             assert np.max(np.abs(tb[fin] - ref[fin])) <= 1e-4 * ref[fin].max()
             assert np.max(np.abs(tb[fin] - single[fin])) <= 1e-4 * ref[fin].max()
+
+
+def test_graph_replay_reused_across_solves():
+    """launch_mode 2 captures one chunk of waves as a graph and replays it; a second solve
+    with the same buffers reuses the graph, a third with other sizes rebuilds it."""
+    speed = g.gen_smooth_speed(640, 384, 61, 8.0, 0.1, 10.0)
+    seeds = g.gen_points(640, 384, 62, 3)
+    cost = g.cost_from_speed(speed)
+    ref = oracle.fmm_f32(cost, seeds)
+    small = cost[:200, :300].copy()
+    ref_small = oracle.fmm_f32(small, [[5, 7]])
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=64, inner_iter_cap=1,
+                   launch_mode=2, waves_per_poll=4) as ctx:
+        dc = ctx.to_device(cost)
+        dt = ctx.alloc(cost.shape, np.float32)
+        for rep in range(3):
+            print(f"case graph replay rep={rep}", flush=True)
+            st = ctx.solve_device(dc.ptr, g.FIELD_COST_F32, 640, 384, seeds, dt.ptr)
+            assert st["converged"] == 1
+            t = dt.download()
+            assert np.max(np.abs(t.astype(np.float64) - ref)) <= 1e-4 * ref.max()
+        t2, st2 = ctx.solve(small, [[5, 7]])
+        assert np.max(np.abs(t2.astype(np.float64) - ref_small)) <= 1e-4 * ref_small.max()
+
+
+def test_autotune_picks_fastest_converged():
+    speed = g.gen_smooth_speed(512, 512, 71, 16.0, 0.1, 10.0)
+    seeds = g.gen_points(512, 512, 72, 2)
+    cost = g.cost_from_speed(speed)
+    ref = oracle.fmm_f32(cost, seeds)
+    cands = [dict(tile_w=16, tile_h=16, kernel=g.KERNEL_SYSTOLIC, n_active=256),
+             dict(tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=64, inner_iter_cap=1),
+             dict(tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=64, inner_iter_cap=1,
+                  launch_mode=2),
+             dict(tile_w=8, tile_h=8, kernel=g.KERNEL_FIM, n_active=1024)]
+    with g.Context(0) as ctx:
+        dc = ctx.to_device(cost)
+        dt = ctx.alloc(cost.shape, np.float32)
+        print("case autotune 4 candidates", flush=True)
+        best, res = ctx.autotune(g.FIELD_COST_F32, [(dc.ptr, dt.ptr, 512, 512, seeds)], cands, reps=2)
+        for r in res:
+            print(f"  {r}", flush=True)
+        conv = [r for r in res if r["converged"]]
+        assert len(conv) == len(cands)
+        assert best["device_us"] == min(r["device_us"] for r in conv)
+        c = ctx.config
+        assert (c.tile_w, c.tile_h, c.kernel, c.n_active) == (best["tile_w"], best["tile_h"],
+                                                               best["kernel"], best["n_active"])
+        t, _ = ctx.solve(cost, seeds)
+        assert np.max(np.abs(t.astype(np.float64) - ref)) <= 1e-4 * ref.max()

@@ -1,0 +1,201 @@
+"""Continuous-batching server (ghcm_serve_*) on the GPU: results alone vs in the mix,
+admission by HBM budget, errors, graph replay, threads.  Needs an MI355X."""
+import threading
+
+import numpy as np
+import pytest
+
+import ghcm_amd as g
+import oracle
+from ghcm_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+SWEEP = dict(tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512, inner_iter_cap=1,
+             waves_per_poll=8)
+SYSTOLIC = dict(tile_w=16, tile_h=16, kernel=g.KERNEL_SYSTOLIC, n_active=2048, waves_per_poll=8)
+
+
+@pytest.fixture(scope="module")
+def mix():
+    """12 uneven requests: sides 64..1024 (ragged), ring or smooth, 1-4 sources."""
+    sides = [(64, 64), (100, 37), (256, 256), (1024, 512), (130, 700), (512, 512),
+             (64, 1024), (768, 320), (333, 333), (1024, 1024), (200, 90), (640, 640)]
+    out = []
+    for i, (w, h) in enumerate(sides):
+        kind = "ring" if i % 2 == 0 else "smooth"
+        out.append(workloads._make(w, h, kind, 9000 + 2 * i, 1 + i % 4, 9001 + 2 * i))
+    return out
+
+
+def _serve_all(server, reqs, order=None):
+    order = range(len(reqs)) if order is None else order
+    tickets = {i: server.submit(reqs[i].cost, reqs[i].seeds) for i in order}
+    return {i: server.wait(t, 60.0) for i, t in tickets.items()}
+
+
+@pytest.mark.parametrize("cfg", [SWEEP, SYSTOLIC], ids=["sweep64", "systolic16"])
+def test_alone_equals_mix_and_oracle(mix, cfg):
+    print(f"case serve alone-vs-mix cfg={cfg}", flush=True)
+    with g.Server(0, **cfg) as srv:
+        alone = {}
+        for i, r in enumerate(mix):
+            alone[i] = srv.wait(srv.submit(r.cost, r.seeds), 60.0)
+        mixed = _serve_all(srv, mix)
+        st = srv.stats()
+    assert st["completed"] == 2 * len(mix)
+    assert st["peak_live"] > 1, "requests were never in flight together"
+    worst = 0.0
+    for i, r in enumerate(mix):
+        ta, ra = alone[i]
+        tm, rm = mixed[i]
+        assert ra["status"] == g.OK and rm["status"] == g.OK
+        ref = oracle.fmm_f32(r.cost, r.seeds)
+        fin = np.isfinite(ref)
+        assert np.array_equal(np.isfinite(ta), fin) and np.array_equal(np.isfinite(tm), fin)
+        scale = float(ref[fin].max())
+        for t in (ta, tm):
+            assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * scale
+        d = float(np.max(np.abs(ta[fin].astype(np.float64) - tm[fin])))
+        worst = max(worst, d / scale)
+        print(f"  req {i} {r.w}x{r.h} alone-vs-mix linf/max={d / scale:.2e} "
+              f"bitwise={np.array_equal(ta, tm)} latency_alone={ra['latency_us']:.0f}us "
+              f"latency_mix={rm['latency_us']:.0f}us", flush=True)
+    # Alone and in the mix the same tiles converge to the same fixed point; only the
+    # order of tile visits differs, which moves times by rounding-level amounts.
+    assert worst <= 2e-6, worst
+
+
+def test_mix_in_reverse_order_and_graph_mode(mix):
+    print("case serve graph launch_mode=2 reverse order", flush=True)
+    with g.Server(0, launch_mode=2, **SWEEP) as srv:
+        res = _serve_all(srv, mix, order=list(reversed(range(len(mix)))))
+    for i, r in enumerate(mix):
+        t, info = res[i]
+        ref = oracle.fmm_f32(r.cost, r.seeds)
+        fin = np.isfinite(ref)
+        assert np.array_equal(np.isfinite(t), fin)
+        assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+
+
+def test_budget_limits_admission(mix):
+    """A budget of ~2 MiB admits the small requests together, the large ones alone."""
+    budget = 2 << 20
+    small = [r for r in mix if r.w * r.h * 8 + 512 <= budget]
+    print(f"case serve budget={budget} n={len(small)}", flush=True)
+    with g.Server(0, hbm_budget_bytes=budget, **SWEEP) as srv:
+        res = _serve_all(srv, small)
+        st = srv.stats()
+    assert st["peak_bytes"] <= budget
+    assert st["bytes_in_use"] == 0
+    for i, r in enumerate(small):
+        t, info = res[i]
+        ref = oracle.fmm_f32(r.cost, r.seeds)
+        fin = np.isfinite(ref)
+        assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+        assert info["queue_us"] >= 0.0 and info["latency_us"] >= info["queue_us"]
+
+
+def test_slot_limit_and_reuse(mix):
+    """Two slots: twelve requests pass through the same two slots and tile ranges."""
+    print("case serve max_slots=2", flush=True)
+    with g.Server(0, max_slots=2, **SWEEP) as srv:
+        res = _serve_all(srv, mix)
+        st = srv.stats()
+    assert st["peak_live"] <= 2
+    for i, r in enumerate(mix):
+        t, _ = res[i]
+        ref = oracle.fmm_f32(r.cost, r.seeds)
+        fin = np.isfinite(ref)
+        assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+
+
+def test_errors_and_edge_requests():
+    print("case serve errors", flush=True)
+    with g.Server(0, hbm_budget_bytes=1 << 20, **SWEEP) as srv:
+        big = np.ones((1024, 1024), np.float32)
+        with pytest.raises(g.GhcmError) as e:
+            srv.submit(big, [[0, 0]])
+        assert e.value.status == g.ERR_OVER_BUDGET
+        with pytest.raises(g.GhcmError) as e:
+            srv.submit(np.ones((8, 8), np.float32), [[8, 0]])
+        assert e.value.status == g.ERR_INVALID_ARG
+        with pytest.raises(g.GhcmError) as e:
+            srv.wait(12345, 0.0)
+        assert e.value.status == g.ERR_NOT_FOUND
+        # no seeds: every time stays +INF; a 1x1 field with its seed: one zero
+        t0 = srv.submit(np.ones((50, 70), np.float32), np.zeros((0, 2), np.uint32))
+        t1 = srv.submit(np.ones((1, 1), np.float32), [[0, 0]])
+        a, _ = srv.wait(t0, 30.0)
+        b, _ = srv.wait(t1, 30.0)
+        assert np.all(np.isinf(a))
+        assert b.shape == (1, 1) and b[0, 0] == 0.0
+    with pytest.raises(g.GhcmError):
+        g.Server(0, max_slots=5000, **SWEEP)
+    with pytest.raises(g.GhcmError):
+        g.Server(0, scheduler=g.SCHED_HOST_PQ, **SWEEP)
+
+
+def test_wait_timeout_then_result():
+    print("case serve wait timeout", flush=True)
+    r = workloads._make(2048, 2048, "ring", 31, 1, 32)
+    with g.Server(0, **SWEEP) as srv:
+        t = srv.submit(r.cost, r.seeds)
+        try:
+            srv.wait(t, 0.0)
+            timed_out = False
+        except g.GhcmError as e:
+            assert e.status == g.ERR_TIMEOUT
+            timed_out = True
+        if timed_out:
+            out, info = srv.wait(t, 60.0)
+            assert info["status"] == g.OK and out[r.seeds[0][1], r.seeds[0][0]] == 0.0
+
+
+def test_threads_submit_concurrently(mix):
+    print("case serve 4 submitting threads", flush=True)
+    with g.Server(0, **SWEEP) as srv:
+        results = {}
+        errors = []
+
+        def worker(k):
+            try:
+                for i in range(k, len(mix), 4):
+                    t = srv.submit(mix[i].cost, mix[i].seeds)
+                    results[i] = srv.wait(t, 60.0)
+            except Exception as exc:  # reported below
+                errors.append(exc)
+
+        ths = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    assert not errors, errors
+    assert sorted(results) == list(range(len(mix)))
+    for i, r in enumerate(mix):
+        t, _ = results[i]
+        ref = oracle.fmm_f32(r.cost, r.seeds)
+        fin = np.isfinite(ref)
+        assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+
+
+def test_reduced_precision_server():
+    print("case serve speed_f16", flush=True)
+    speed = g.gen_smooth_speed(300, 200, 41, 8.0, 0.1, 10.0)
+    seeds = g.gen_points(300, 200, 42, 2)
+    ref = oracle.fmm_f64(1.0 / speed, seeds)
+    with g.Server(0, kind=g.FIELD_SPEED_F16, **SWEEP) as srv:
+        t, _ = srv.wait(srv.submit(speed.astype(np.float16), seeds), 30.0)
+    assert np.max(np.abs(t - ref)) <= 1e-3 * ref.max()
+
+
+def test_stop_without_drain_returns():
+    print("case serve stop(drain=False)", flush=True)
+    r = workloads._make(1024, 1024, "smooth", 51, 2, 52)
+    srv = g.Server(0, **SWEEP)
+    for _ in range(6):
+        srv.submit(r.cost, r.seeds)
+    srv.stop(drain=False)
+    with pytest.raises(g.GhcmError):
+        srv.submit(r.cost, r.seeds)
