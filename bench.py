@@ -143,79 +143,116 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             "max_rel_diff_vs_plain": worst}
 
 
-def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256) -> dict:
-    """Continuous batching: saturated throughput, then Poisson arrivals for p50/p99."""
+def _serve_runs(srv, submit, n_requests: int, seed: int) -> dict:
+    """Saturated run (every request submitted at once), then Poisson arrivals at half the
+    saturated rate; latencies are submit -> result, measured by the library."""
     import threading
 
     import numpy as np
+    for _ in range(2):          # the first pass grows the server's pinned staging pool
+        t0 = time.perf_counter()
+        tickets = [submit(i) for i in range(n_requests)]
+        results = [srv.wait(t) for t in tickets]
+        t_sat = time.perf_counter() - t0
+    sat_rate = n_requests / t_sat
+    rate = 0.5 * sat_rate
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, n_requests)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    tickets = [None] * n_requests
+
+    def client():
+        start = time.perf_counter()
+        for i in range(n_requests):
+            d = arrivals[i] - (time.perf_counter() - start)
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = submit(i)
+
+    th = threading.Thread(target=client)
+    t0 = time.perf_counter()
+    th.start()
+    lat = []
+    for i in range(n_requests):
+        while tickets[i] is None:
+            time.sleep(0.0002)
+        _, info = srv.wait(tickets[i])
+        lat.append(info["latency_us"] / 1e3)
+    th.join()
+    t_open = time.perf_counter() - t0
+    lat = np.array(lat)
+    return {"results": results,
+            "saturated_solves_per_s": round(sat_rate, 1),
+            "poisson_rate_per_s": round(rate, 1),
+            "poisson_achieved_per_s": round(n_requests / t_open, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3),
+            "peak_live": srv.stats()["peak_live"]}
+
+
+def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256) -> dict:
+    """Continuous batching on requests already in HBM (device buffers) and on host buffers
+    (PCIe in and out), each against the same solves made one at a time."""
+    import numpy as np
     from ghcm_amd import workloads
     _, insts = workloads.serving_requests(n_distinct, 1.0)
-    reqs = [insts[i % n_distinct] for i in range(n_requests)]
-    # plain: one ghcm_solve (host buffers in and out) after another
+    pick = [i % n_distinct for i in range(n_requests)]
+
+    def rel_diff(t, p):
+        fin = np.isfinite(p)
+        if not np.array_equal(fin, np.isfinite(t)):
+            return float("inf")
+        return float(np.max(np.abs(t[fin] - p[fin])) / p[fin].max()) if fin.any() else 0.0
+
+    out = {"workload": f"{n_requests} requests cycling {n_distinct} square fields, sides "
+                       "log-uniform 128..2048, ring or smooth (sigma 8), 1-4 sources (seed 5050)"}
     with g.Context(0, **serve_cfg) as ctx:
+        # ---- device-resident requests ----
+        d_fields = [ctx.to_device(r.cost) for r in insts]
+        d_plain = [ctx.alloc((r.h, r.w), np.float32) for r in insts]
+        d_times = [ctx.alloc((insts[k].h, insts[k].w), np.float32) for k in pick]
+        for k in range(4):
+            r = insts[k]
+            ctx.solve_device(d_fields[k].ptr, g.FIELD_COST_F32, r.w, r.h, r.seeds, d_plain[k].ptr)
+        t0 = time.perf_counter()
+        for k, r in enumerate(insts):
+            ctx.solve_device(d_fields[k].ptr, g.FIELD_COST_F32, r.w, r.h, r.seeds, d_plain[k].ptr)
+        t_plain_dev = (time.perf_counter() - t0) / n_distinct
+        plain = [d.download() for d in d_plain]
+        with g.Server(0, **serve_cfg) as srv:
+            def submit_dev(i):
+                k = pick[i]
+                r = insts[k]
+                return srv.submit_device(d_fields[k].ptr, r.w, r.h, r.seeds, d_times[i].ptr)
+            for i in range(4):
+                srv.wait(submit_dev(i))
+            dev = _serve_runs(srv, submit_dev, n_requests, 5051)
+        dev.pop("results")
+        worst = max(rel_diff(d_times[i].download(), plain[pick[i]]) for i in range(n_requests))
+        dev.update({"plain_solves_per_s": round(1.0 / t_plain_dev, 1),
+                    "speedup_vs_plain": round(dev["saturated_solves_per_s"] * t_plain_dev, 2),
+                    "max_rel_diff_vs_plain": worst})
+        out["device_resident"] = dev
+        for d in d_fields + d_plain + d_times:
+            d.free()
+        # ---- host buffers: PCIe in and out ----
         for r in insts[:4]:
             ctx.solve(r.cost, r.seeds)
-        plain = {}
         t0 = time.perf_counter()
-        for i, r in enumerate(insts):
-            plain[i], _ = ctx.solve(r.cost, r.seeds)
-        t_plain = (time.perf_counter() - t0) / n_distinct
-    out = {"workload": f"{n_requests} requests cycling {n_distinct} square fields, sides "
-                       "log-uniform 128..2048, ring or smooth, 1-4 sources (seed 5050), host "
-                       "buffers in and out",
-           "plain_solves_per_s": round(1.0 / t_plain, 1)}
+        for r in insts:
+            ctx.solve(r.cost, r.seeds)
+        t_plain_host = (time.perf_counter() - t0) / n_distinct
     with g.Server(0, **serve_cfg) as srv:
-        for r in insts[:4]:
-            srv.wait(srv.submit(r.cost, r.seeds))
-        t0 = time.perf_counter()
-        tickets = [srv.submit(r.cost, r.seeds) for r in reqs]
-        res = [srv.wait(t) for t in tickets]
-        t_sat = time.perf_counter() - t0
-        worst = 0.0
-        for i, (t, info) in enumerate(res):
-            p = plain[i % n_distinct]
-            fin = np.isfinite(p)
-            if fin.any():
-                worst = max(worst, float(np.max(np.abs(t[fin] - p[fin])) / p[fin].max()))
-        sat_rate = n_requests / t_sat
-        # open loop at half the saturated rate
-        rate = 0.5 * sat_rate
-        gaps = np.random.default_rng(5051).exponential(1.0 / rate, n_requests)
-        arrivals = np.cumsum(gaps) - gaps[0]
-        tickets = [None] * n_requests
-        submitted = threading.Event()
-
-        def client():
-            start = time.perf_counter()
-            for i, r in enumerate(reqs):
-                d = arrivals[i] - (time.perf_counter() - start)
-                if d > 0:
-                    time.sleep(d)
-                tickets[i] = srv.submit(r.cost, r.seeds)
-            submitted.set()
-
-        th = threading.Thread(target=client)
-        t0 = time.perf_counter()
-        th.start()
-        lat = []
-        for i in range(n_requests):
-            while tickets[i] is None:
-                time.sleep(0.0002)
-            _, info = srv.wait(tickets[i])
-            lat.append(info["latency_us"] / 1e3)
-        th.join()
-        t_open = time.perf_counter() - t0
-        sst = srv.stats()
-    lat = np.array(lat)
-    out.update({"saturated_solves_per_s": round(sat_rate, 1),
-                "speedup_vs_plain": round(sat_rate * t_plain, 2),
-                "poisson_rate_per_s": round(rate, 1),
-                "poisson_achieved_per_s": round(n_requests / t_open, 1),
-                "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                "max_ms": round(float(lat.max()), 3),
-                "peak_live": sst["peak_live"],
-                "max_rel_diff_vs_plain": worst})
+        costs = [r.cost for r in insts]
+        for i in range(4):
+            srv.wait(srv.submit(costs[i], insts[i].seeds))
+        host = _serve_runs(srv, lambda i: srv.submit(costs[pick[i]], insts[pick[i]].seeds),
+                           n_requests, 5052)
+    worst = max(rel_diff(t, plain[pick[i]]) for i, (t, _) in enumerate(host.pop("results")))
+    host.update({"plain_solves_per_s": round(1.0 / t_plain_host, 1),
+                 "speedup_vs_plain": round(host["saturated_solves_per_s"] * t_plain_host, 2),
+                 "max_rel_diff_vs_plain": worst})
+    out["host_buffers"] = host
     return out
 
 
@@ -340,7 +377,12 @@ def main() -> int:
         if "autotuned" in out.get("batched", {}):
             scfg.update(out["batched"]["autotuned"])
         out["serving"] = bench_serving(g, scfg)
-        log(f"bench: serving p50 {out['serving']['p50_ms']} ms p99 {out['serving']['p99_ms']} ms")
+        sv = out["serving"]
+        log(f"bench: serving (HBM-resident) {sv['device_resident']['saturated_solves_per_s']} "
+            f"solves/s, {sv['device_resident']['speedup_vs_plain']}x plain, p50 "
+            f"{sv['device_resident']['p50_ms']} ms p99 {sv['device_resident']['p99_ms']} ms; "
+            f"host buffers {sv['host_buffers']['saturated_solves_per_s']} solves/s, "
+            f"{sv['host_buffers']['speedup_vs_plain']}x plain")
 
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline ({args.cpu_trials} trials of the oracle FMM)")

@@ -122,7 +122,8 @@ EXPORTED_SYMBOLS = (
     "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
     "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download", "ghcm_dev_copy",
     "ghcm_solve_batch", "ghcm_tune_default_space", "ghcm_autotune",
-    "ghcm_serve_config_default", "ghcm_serve_start", "ghcm_serve_submit", "ghcm_serve_wait",
+    "ghcm_serve_config_default", "ghcm_serve_start", "ghcm_serve_submit",
+    "ghcm_serve_submit_device", "ghcm_serve_wait",
     "ghcm_serve_get_stats", "ghcm_serve_stop",
     "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
     "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
@@ -168,6 +169,7 @@ def lib() -> ctypes.CDLL:
     L.ghcm_serve_config_default.restype = None
     L.ghcm_serve_start.argtypes = [i32, P(Config), P(ServeConfig), P(vp)]
     L.ghcm_serve_submit.argtypes = [vp, vp, u32, u32, vp, u32, vp, P(ctypes.c_uint64)]
+    L.ghcm_serve_submit_device.argtypes = [vp, vp, u32, u32, vp, u32, vp, P(ctypes.c_uint64)]
     L.ghcm_serve_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_double, P(ServeResult)]
     L.ghcm_serve_get_stats.argtypes = [vp, P(ServeStats)]
     L.ghcm_serve_stop.argtypes = [vp, i32]
@@ -383,8 +385,8 @@ class Server:
     """Continuous-batching solve server (ghcm_serve_*): one device wave loop that admits
     requests at wave boundaries while earlier ones are still being solved.
 
-    ``submit`` returns a ticket; the field array and the output array are kept alive by
-    this object until ``wait`` collects the ticket."""
+    ``submit`` (host arrays; the field is copied before it returns) and ``submit_device``
+    (device pointers, results written in place) return a ticket; ``wait`` collects it."""
 
     def __init__(self, ordinal: int = 0, kind: int = FIELD_COST_F32, hbm_budget_bytes: int = 0,
                  max_slots: int = 0, max_tiles: int = 0, **config):
@@ -417,16 +419,26 @@ class Server:
         t = ctypes.c_uint64(0)
         _check(self._L.ghcm_serve_submit(self._h, f.ctypes.data, w, h, s.ctypes.data, s.shape[0],
                                          out.ctypes.data, ctypes.byref(t)))
-        self._held[t.value] = (f, out)
+        self._held[t.value] = out
+        return t.value
+
+    def submit_device(self, d_field: int, w: int, h: int, seeds, d_time: int) -> int:
+        """Device-resident request: raw device pointers on the server's device."""
+        s = _seeds_array(seeds)
+        t = ctypes.c_uint64(0)
+        _check(self._L.ghcm_serve_submit_device(self._h, ctypes.c_void_p(d_field), w, h,
+                                                s.ctypes.data, s.shape[0], ctypes.c_void_p(d_time),
+                                                ctypes.byref(t)))
         return t.value
 
     def wait(self, ticket: int, timeout_s: float = -1.0):
-        """(times (H, W) float32, result dict); raises GhcmError on failure or time-out."""
+        """(times (H, W) float32 or None for a device request, result dict); raises
+        GhcmError on failure or time-out."""
         r = ServeResult()
         rc = self._L.ghcm_serve_wait(self._h, ticket, timeout_s, ctypes.byref(r))
         if rc == ERR_TIMEOUT:
             _check(rc)
-        f, out = self._held.pop(ticket, (None, None))
+        out = self._held.pop(ticket, None)
         _check(rc)
         return out, r.to_dict()
 

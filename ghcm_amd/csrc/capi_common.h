@@ -35,6 +35,22 @@ inline uint32_t bytes_per_visit(uint32_t tw, uint32_t th, int kind) {
             return ::ghcm_capi::fail(GHCM_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
+// Waits for `ev` by polling it.  A blocking wait may sleep for tens of microseconds past
+// the GPU's signal, which a wave loop polled a few times per solve pays every time.
+inline hipError_t spin_wait(hipEvent_t ev) {
+    for (;;) {
+        const hipError_t q = hipEventQuery(ev);
+        if (q != hipErrorNotReady) return q;
+        __builtin_ia32_pause();
+    }
+}
+
+// Records `ev` on `st` and polls it: the stream's work so far has finished on return.
+inline hipError_t spin_sync(hipStream_t st, hipEvent_t ev) {
+    const hipError_t e = hipEventRecord(ev, st);
+    return e != hipSuccess ? e : spin_wait(ev);
+}
+
 // Device buffer that only grows; reused across solves.
 struct DevBuf {
     void* p = nullptr;
@@ -61,19 +77,21 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Pinned host buffer that only grows.
+// Pinned host buffer that only grows: mapped into the device's address space and
+// coherent, so kernels read and write it directly (`dev` is the device view).
 struct PinnedBuf {
     void* p = nullptr;
+    void* dev = nullptr;
     size_t bytes = 0;
     int reserve(size_t want) {
         if (want <= bytes) return GHCM_OK;
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        bytes = 0;
+        release();
         size_t b = std::max<size_t>(want, 256);
-        hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, b, hipHostMallocMapped | hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(&dev, p, 0);
         if (e != hipSuccess) {
-            p = nullptr;
+            if (p) (void)hipHostFree(p);
+            p = dev = nullptr;
             return fail(GHCM_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", b, hipGetErrorString(e));
         }
         bytes = b;
@@ -81,10 +99,11 @@ struct PinnedBuf {
     }
     void release() {
         if (p) (void)hipHostFree(p);
-        p = nullptr;
+        p = dev = nullptr;
         bytes = 0;
     }
     template <class T> T* as() const { return static_cast<T*>(p); }
+    template <class T> T* dev_as() const { return static_cast<T*>(dev); }
 };
 
 }  // namespace ghcm_capi

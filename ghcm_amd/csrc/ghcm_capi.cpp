@@ -53,6 +53,7 @@ struct ghcm_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_sync = nullptr;       // polled in place of stream synchronisation
     std::vector<hipEvent_t> ev_prof;    // profile_kernels: begin/end pairs per queued wave
     std::vector<hipEvent_t> ev_slot;    // device scheduler: one per chunk in flight
     // launch_mode 2: one chunk of waves captured as a HIP graph, reused while the solve
@@ -62,7 +63,9 @@ struct ghcm_ctx {
     // device scratch
     DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals, sel;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
-    PinnedBuf h_ctrl, h_init, h_seeds, h_inst, h_jobs, h_status;
+    PinnedBuf h_ctrl, h_seeds, h_inst, h_jobs, h_status;
+    PinnedBuf rec;                       // HostRec written by the wave loop
+    uint32_t solve_seq = 0;              // tags the totals of each solve
 };
 
 // ---------------------------------------------------------------------------------------
@@ -187,6 +190,7 @@ extern "C" int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** o
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_begin);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev_end);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming);
     if (e != hipSuccess) {
         ghcm_ctx_destroy(ctx);
         return fail(GHCM_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
@@ -205,11 +209,12 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
                       &ctx->tstat, &ctx->totals, &ctx->sel,
                       &ctx->field_io, &ctx->time_io})
         b->release();
-    for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_init, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
-                         &ctx->h_status})
+    for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
+                         &ctx->h_status, &ctx->rec})
         b->release();
     if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
+    if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
     for (hipEvent_t ev : ctx->ev_prof)
         if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : ctx->ev_slot)
@@ -305,10 +310,9 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
     if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
     if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
-    if ((rc = ctx->seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
-    if ((rc = ctx->h_init.reserve(sizeof(GhcmCtrl)))) return rc;
-    if ((rc = ctx->h_ctrl.reserve(4 * sizeof(GhcmCtrl) + 64))) return rc;
+    if ((rc = ctx->h_ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
+    if ((rc = ctx->rec.reserve(sizeof(HostRec)))) return rc;
 
     uint32_t* hs = ctx->h_seeds.as<uint32_t>();
     {
@@ -327,9 +331,11 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     // each work-group loops over list entries, so the grid never has to match the list.
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
     const uint32_t grid = std::max(1u, std::min(n_sel, resident));
-    GhcmCtrl* init = ctx->h_init.as<GhcmCtrl>();
-    std::memset(init, 0, sizeof *init);
-    init->n_select = n_sel;
+    uint64_t max_cells = 0;
+    for (uint32_t i = 0; i < n_inst; ++i) max_cells = std::max<uint64_t>(max_cells, (uint64_t)hi[i].w * hi[i].h);
+    volatile HostRec* hr = ctx->rec.as<HostRec>();
+    hr->ticks = 0;
+    hr->done = 0;
 
     SolveArgs a{};
     a.inst = ctx->inst.as<GhcmInst>();
@@ -348,22 +354,35 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
 
+    // The instance table and the seeds are read by the kernels from mapped pinned memory:
+    // the whole solve issues no runtime copy.
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
-    HIP_TRY(hipMemcpyAsync(a.ctrl, init, sizeof(GhcmCtrl), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(ctx->inst.p, hinst, sizeof(GhcmInst) * n_inst, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(ctx->seeds.p, hs, 12ull * n_seeds_total, hipMemcpyHostToDevice, st));
-    for (uint32_t i = 0; i < n_inst; ++i)
-        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(hi[i].d_time), (size_t)hi[i].w * hi[i].h, kInfBits, st));
-    HIP_TRY(launch_fill_u32(a.qwave, T, 0u, st));
-    HIP_TRY(launch_fill_u32(a.key, 2ull * T, kInfBits, st));
-    HIP_TRY(launch_fill_u32(a.mask, 2ull * T, 0u, st));
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(a.tmin), T, kInfBits, st));
-    HIP_TRY(launch_fill_u32(a.tstat, 3ull * T, 0u, st));
-    HIP_TRY(launch_fill_u32(ctx->totals.as<uint32_t>(), 6, 0u, st));
-    if (n_inst > 1) HIP_TRY(launch_tile_inst(ctx->tinst.as<uint32_t>(), a.inst, n_inst, st));
-    HIP_TRY(launch_seed(a, ctx->seeds.as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
+    HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel,
+                              ctx->totals.as<unsigned long long>(), st));
+    HIP_TRY(launch_seed(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
 
     uint64_t max_waves = cfg.max_waves ? cfg.max_waves : (uint64_t)8 * T + 100000;
+    // Per-tile counters are summed once after the last wave (outside the timed span) and
+    // handed over through the progress record; the host polls for this solve's tag.
+    bool totals_ready = false;
+    auto publish_totals = [&]() -> int {
+        const uint32_t tag = ++ctx->solve_seq;
+        HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
+        HIP_TRY(launch_publish_totals(ctx->totals.as<unsigned long long>(), ctx->rec.dev_as<HostRec>(), tag, st));
+        for (uint64_t spins = 0; hr->totals_tag != tag; ++spins) {
+            if ((spins & 4095u) == 4095u) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    return fail(GHCM_ERR_HIP, "solve: %s", hipGetErrorString(q));
+                if (q == hipSuccess && hr->totals_tag != tag)
+                    return fail(GHCM_ERR_HIP, "solve: the progress record was not written");
+            }
+            __builtin_ia32_pause();
+        }
+        HIP_TRY(ghcm_capi::spin_wait(ctx->ev_end));
+        totals_ready = true;
+        return GHCM_OK;
+    };
     bool converged = false;
     uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0, kern_launches = 0;
     double kern_us = 0.0;
@@ -401,55 +420,31 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         converged = hc[0].done != 0;
         waves_done = hc[0].busy_waves;
     } else if (!host_sched) {
-        // ---- device-resident wave loop; the host only polls a done flag ----
-        // Fixed chunks of `chunk` waves, up to kSlots chunks in flight: the GPU always has
-        // queued work while the host polls, and after the last busy wave at most
-        // kSlots * chunk no-op waves (each a cheap early-exit launch) remain queued.
+        // ---- device-resident wave loop ----
+        // k_wave_end publishes a tick, the done flag and the busy-wave count into the
+        // mapped progress record on every launch.  The host keeps `lead` waves queued
+        // ahead of the last executed boundary and stops as soon as it reads `done`, so at
+        // most `lead` idle waves (two cheap early-exit launches each) follow the last
+        // busy one, and no runtime copy or blocking wait sits in the loop.
+        a.rec = ctx->rec.dev_as<HostRec>();
         HIP_TRY(launch_wave_end(a, 1u, st));
-        constexpr int kSlots = 4;
-        GhcmCtrl* hc = ctx->h_ctrl.as<GhcmCtrl>();
-        const uint32_t chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
         const bool prof = cfg.profile_kernels != 0;
-        while (ctx->ev_slot.size() < (size_t)kSlots) {
-            hipEvent_t ev;
-            HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            ctx->ev_slot.push_back(ev);
-        }
-        if (prof) {
-            const size_t need = 2ull * kSlots * chunk;
-            while (ctx->ev_prof.size() < need) {
-                hipEvent_t ev;
-                HIP_TRY(hipEventCreate(&ev));
-                ctx->ev_prof.push_back(ev);
-            }
-        }
-        uint32_t chunk_len[kSlots] = {0, 0, 0, 0};
-        auto harvest = [&](int sl) -> int {        // add the slot's launch times
-            for (uint32_t k = 0; k < chunk_len[sl]; ++k) {
-                float ms = 0.0f;
-                hipEvent_t b = ctx->ev_prof[((size_t)sl * chunk + k) * 2];
-                hipEvent_t e = ctx->ev_prof[((size_t)sl * chunk + k) * 2 + 1];
-                HIP_TRY(hipEventElapsedTime(&ms, b, e));
-                kern_us += 1000.0 * ms;
-                ++kern_launches;
-            }
-            chunk_len[sl] = 0;
-            return GHCM_OK;
-        };
+        const uint32_t lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);
         const bool use_graph = cfg.launch_mode == 2 && !prof;
+        const uint32_t step = use_graph ? std::max<uint32_t>(1, lead / 2) : 1;
         if (use_graph) {
-            // key: the kernel arguments and launch shape of one chunk
+            // key: the kernel arguments and launch shape of one captured step of waves
             std::vector<unsigned char> key(sizeof(SolveArgs) + 2 * sizeof(uint32_t));
             std::memcpy(key.data(), &a, sizeof(SolveArgs));
             std::memcpy(key.data() + sizeof(SolveArgs), &grid, sizeof(uint32_t));
-            std::memcpy(key.data() + sizeof(SolveArgs) + sizeof(uint32_t), &chunk, sizeof(uint32_t));
+            std::memcpy(key.data() + sizeof(SolveArgs) + sizeof(uint32_t), &step, sizeof(uint32_t));
             if (!ctx->gexec || key != ctx->gkey) {
                 if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
                 ctx->gexec = nullptr;
                 hipGraph_t graph = nullptr;
                 HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
                 hipError_t ce = hipSuccess;
-                for (uint32_t k = 0; k < chunk && ce == hipSuccess; ++k) {
+                for (uint32_t k = 0; k < step && ce == hipSuccess; ++k) {
                     ce = launch(a, grid, st, nullptr, nullptr);
                     if (ce == hipSuccess) ce = launch_wave_end(a, 0u, st);
                 }
@@ -466,43 +461,50 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
                 ctx->gkey = key;
             }
         }
-        uint64_t enqueued = 0;
-        int inflight = 0, head = 0;                 // head: next slot to fill
-        for (;;) {
-            if (use_graph) {
-                HIP_TRY(hipGraphLaunch(ctx->gexec, st));
-            } else {
-                for (uint32_t k = 0; k < chunk; ++k) {
-                    if (prof)
-                        HIP_TRY(launch(a, grid, st, ctx->ev_prof[((size_t)head * chunk + k) * 2],
-                                       ctx->ev_prof[((size_t)head * chunk + k) * 2 + 1]));
-                    else
-                        HIP_TRY(launch(a, grid, st, nullptr, nullptr));
+        uint64_t enqueued = 0;                      // waves queued after the first boundary
+        for (uint64_t spins = 0;; ++spins) {
+            if (hr->done) { converged = true; break; }
+            if (enqueued >= max_waves) break;
+            const uint32_t ticks = hr->ticks;
+            const uint64_t executed = ticks ? ticks - 1u : 0u;
+            if (enqueued < executed + lead) {
+                if (use_graph) {
+                    HIP_TRY(hipGraphLaunch(ctx->gexec, st));
+                } else if (prof) {
+                    const size_t need = 2 * (enqueued + 1);
+                    while (ctx->ev_prof.size() < need) {
+                        hipEvent_t ev;
+                        HIP_TRY(hipEventCreate(&ev));
+                        ctx->ev_prof.push_back(ev);
+                    }
+                    HIP_TRY(launch(a, grid, st, ctx->ev_prof[2 * enqueued], ctx->ev_prof[2 * enqueued + 1]));
+                    HIP_TRY(launch_wave_end(a, 0u, st));
+                } else {
+                    HIP_TRY(launch(a, grid, st, nullptr, nullptr));
                     HIP_TRY(launch_wave_end(a, 0u, st));
                 }
+                enqueued += step;
+                continue;
             }
-            chunk_len[head] = prof ? chunk : 0;
-            enqueued += chunk;
-            HIP_TRY(hipMemcpyAsync(&hc[head], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipEventRecord(ctx->ev_slot[head], st));
-            ++inflight;
-            head = (head + 1) % kSlots;
-            if (inflight == kSlots) {
-                const int oldest = head;            // the slot filled kSlots chunks ago
-                HIP_TRY(hipEventSynchronize(ctx->ev_slot[oldest]));
-                if ((rc = harvest(oldest))) return rc;
-                --inflight;
-                if (hc[oldest].done) { converged = true; break; }
+            if ((spins & 4095u) == 4095u) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    return fail(GHCM_ERR_HIP, "wave loop: %s", hipGetErrorString(q));
             }
-            if (enqueued >= max_waves) break;
+            __builtin_ia32_pause();
         }
         HIP_TRY(hipEventRecord(ctx->ev_end, st));
-        HIP_TRY(hipMemcpyAsync(&hc[0], a.ctrl, sizeof(GhcmCtrl), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (int sl = 0; sl < kSlots; ++sl)
-            if ((rc = harvest(sl))) return rc;
-        converged = hc[0].done != 0;
-        waves_done = hc[0].busy_waves;
+        if ((rc = publish_totals())) return rc;
+        if (prof) {
+            for (uint64_t k = 0; k < enqueued; ++k) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_prof[2 * k], ctx->ev_prof[2 * k + 1]));
+                kern_us += 1000.0 * ms;
+                ++kern_launches;
+            }
+        }
+        converged = hr->done != 0;
+        waves_done = hr->busy_waves;
     } else {
         // ---- host priority queue (src/eikonal_hcm.cpp:343-482) ----
         if ((rc = ctx->jobs.reserve(sizeof(HostJob) * n_sel))) return rc;
@@ -580,7 +582,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             else
                 HIP_TRY(launch(a, nj, st, nullptr, nullptr));
             HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(ghcm_capi::spin_sync(st, ctx->ev_sync));
             if (cfg.profile_kernels) {
                 float ms = 0.0f;
                 HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_prof[0], ctx->ev_prof[1]));
@@ -606,17 +608,12 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         }
         converged = pq.empty();
         HIP_TRY(hipEventRecord(ctx->ev_end, st));
-        HIP_TRY(hipStreamSynchronize(st));
     }
 
-    // Per-tile counters summed once, outside the timed span.
-    HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
-    unsigned long long* ht = reinterpret_cast<unsigned long long*>(ctx->h_ctrl.as<char>() + 4 * sizeof(GhcmCtrl));
-    HIP_TRY(hipMemcpyAsync(ht, ctx->totals.p, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    visits = ht[0];
-    inner = ht[1];
-    improved = ht[2];
+    if (!totals_ready && (rc = publish_totals())) return rc;
+    visits = hr->totals[0];
+    inner = hr->totals[1];
+    improved = hr->totals[2];
     float dev_ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&dev_ms, ctx->ev_begin, ctx->ev_end));
     const auto t_wall1 = std::chrono::steady_clock::now();

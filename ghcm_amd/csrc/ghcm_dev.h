@@ -58,7 +58,8 @@ struct GhcmCtrl {
     uint32_t wave_ready;    // persistent loop: wave index whose selection is published
     uint32_t abort;         // persistent loop: a barrier wait timed out
     uint32_t carried;       // tiles carried over by the top-n selection (all waves)
-    uint32_t pad0[5];
+    uint32_t ticks;         // k_wave_end launches executed, the idle ones included
+    uint32_t pad0[4];
     uint32_t prefix[kShards + 1];   // exclusive prefix of the shard counts of list[cur]
     uint32_t pad1[31];
     uint32_t count[2][kShards * kCounterStride];
@@ -83,6 +84,23 @@ struct HostStatus {
     uint32_t requeue;       // 1 if the visit hit the pass cap before converging
 };
 
+// Progress record in mapped, coherent pinned host memory.  k_wave_end writes it with
+// system-scope stores on every launch; the host polls it instead of copying the control
+// block back or waiting on events, so the wave loop has no runtime copy in it.
+struct HostRec {
+    uint32_t ticks;                 // k_wave_end launches executed
+    uint32_t done;                  // the loop found its list empty
+    uint32_t wave;
+    uint32_t busy_waves;
+    uint32_t totals_tag;            // set by k_publish_totals once totals[] is written
+    uint32_t pad[11];
+    unsigned long long totals[3];   // tile visits, in-tile passes, cells improved
+    unsigned long long pad2[5];
+    // serving: per slot (generation << 32) | tiles queued at the last wave boundary; a
+    // generation is reported only by boundaries after the slot's admission
+    unsigned long long slot[kMaxServeSlots];
+};
+
 struct SolveArgs {
     const GhcmInst* inst;
     uint32_t        n_inst;
@@ -96,8 +114,9 @@ struct SolveArgs {
     uint32_t        shard_cap;
     uint32_t*       tstat;   // [3][tiles]
     uint32_t*       jobs;    // compacted top-n selection, capacity min(n, tiles)
-    uint32_t*       iact;    // serving: queued tiles per instance slot in list[cur]
-    uint32_t        n_iact;  // serving: instance slots (<= kMaxServeSlots)
+    HostRec*        rec;     // device view of the progress record (null: not written)
+    const uint32_t* sgen;    // serving: admission generation per slot
+    uint32_t        n_slots; // serving: instance slots (<= kMaxServeSlots), 0 otherwise
     GhcmCtrl*       ctrl;
     float           eps;
     uint32_t        iter_cap;

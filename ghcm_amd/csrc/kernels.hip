@@ -82,6 +82,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Stores to mapped host memory: system scope, relaxed (each word is read on its own).
+__device__ __forceinline__ void sys_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void sys_store64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Append tile g to list `which` unless it is already there (qwave), in shard g % kShards.
 __device__ __forceinline__ void list_append(const SolveArgs& a, uint32_t which, uint32_t g) {
     const uint32_t sh = g % kShards;
@@ -892,15 +900,25 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     __shared__ uint32_t s_misc[2];
     __shared__ uint32_t sg[kSelectStage];
     __shared__ uint32_t sk[kSelectStage];
-    if (a.ctrl->done) return;
+    const int tid = threadIdx.x;
+    if (a.ctrl->done) {
+        // Idle launch: only the tick advances, so the host still sees progress.
+        if (a.rec && tid == 0) {
+            const uint32_t t = a.ctrl->ticks + 1u;
+            a.ctrl->ticks = t;
+            sys_store(&a.rec->ticks, t);
+        }
+        return;
+    }
     const uint32_t cur0 = a.ctrl->cur;
     wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
-    if (a.iact) {
-        // Serving: queued tiles per instance slot in the new current list (a slot whose
-        // count is 0 has nothing left to solve).  hist doubles as the counter array.
-        __syncthreads();
-        const int tid = threadIdx.x;
-        const uint32_t ns = a.n_iact;
+    if (!a.rec) return;
+    __syncthreads();
+    if (a.n_slots) {
+        // Serving: queued tiles per instance slot in the new current list, tagged with the
+        // slot's admission generation (a count of 0 means nothing is left to solve).
+        // hist doubles as the counter array.
+        const uint32_t ns = a.n_slots;
         for (uint32_t i = tid; i < ns; i += 1024) hist[i] = 0u;
         __syncthreads();
         const uint32_t c = s_prefix[kShards];
@@ -908,8 +926,86 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
         for (uint32_t i = tid; i < c; i += 1024)
             atomicAdd(&hist[a.tinst[list_entry(a, s_prefix, cur, i)]], 1u);
         __syncthreads();
-        for (uint32_t i = tid; i < ns; i += 1024) a.iact[i] = hist[i];
+        for (uint32_t i = tid; i < ns; i += 1024)
+            sys_store64(&a.rec->slot[i], ((unsigned long long)a.sgen[i] << 32) | hist[i]);
     }
+    if (tid == 0) {
+        GhcmCtrl* c = a.ctrl;
+        const uint32_t t = c->ticks + 1u;
+        c->ticks = t;
+        sys_store(&a.rec->wave, c->wave);
+        sys_store(&a.rec->busy_waves, c->busy_waves);
+        sys_store(&a.rec->done, c->done);
+        sys_store(&a.rec->ticks, t);
+    }
+}
+
+// Per-solve initialisation in one launch (grid.y over instances): the control block and
+// totals zeroed, the instance table copied from mapped host memory, every tile's queue
+// state reset and mapped to its instance, every time field set to +INF.
+__global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_inst,
+                             uint32_t n_select, unsigned long long* totals) {
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        uint32_t* c = reinterpret_cast<uint32_t*>(a.ctrl);
+        for (uint32_t i = threadIdx.x; i < sizeof(GhcmCtrl) / 4; i += blockDim.x) c[i] = 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a.ctrl->n_select = n_select;
+            totals[0] = totals[1] = totals[2] = 0ull;
+        }
+    }
+    const size_t T = a.tiles;
+    for (uint32_t j = blockIdx.y; j < n_inst; j += gridDim.y) {
+        const GhcmInst in = h_inst[j];
+        if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<GhcmInst*>(a.inst)[j] = in;
+        const uint32_t nt = in.tiles_x * in.tiles_y;
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+            const size_t g = in.tile_base + t;
+            a.qwave[g] = 0u;
+            a.key[g] = kInfBits;
+            a.key[T + g] = kInfBits;
+            a.mask[g] = 0u;
+            a.mask[T + g] = 0u;
+            a.tmin[g] = __builtin_inff();
+            a.tstat[g] = 0u;
+            a.tstat[T + g] = 0u;
+            a.tstat[2 * T + g] = 0u;
+            if (a.tinst) const_cast<uint32_t*>(a.tinst)[g] = j;
+        }
+        uint32_t* tm = reinterpret_cast<uint32_t*>(in.time);
+        const size_t cells = (size_t)in.w * in.h;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += stride)
+            tm[i] = kInfBits;
+    }
+}
+
+// Totals of the per-tile counters handed to the host record; the tag is written last.
+__global__ void k_publish_totals(const unsigned long long* totals, HostRec* rec, uint32_t tag) {
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < 3; ++j) sys_store64(&rec->totals[j], totals[j]);
+        sys_store(&rec->totals_tag, tag);
+    }
+}
+
+// Bulk copy between device memory and mapped host memory (or device to device), 16 bytes
+// per lane where both ends are 16-byte aligned.  Used in place of runtime copies so no
+// copy engine sits between the host and the wave loop.
+__global__ void k_copy(const void* src, void* dst, size_t bytes) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u) == 0;
+    size_t done = 0;
+    if (vec) {
+        const size_t n16 = bytes / 16;
+        const uint4* s4 = static_cast<const uint4*>(src);
+        uint4* d4 = static_cast<uint4*>(dst);
+        for (size_t i = tid; i < n16; i += stride) d4[i] = s4[i];
+        done = n16 * 16;
+    }
+    const unsigned char* s1 = static_cast<const unsigned char*>(src);
+    unsigned char* d1 = static_cast<unsigned char*>(dst);
+    for (size_t i = done + tid; i < bytes; i += stride) d1[i] = s1[i];
 }
 
 // Serving admission: seeds of newly admitted instances, (x, y, slot) triples.  Queues the
@@ -935,13 +1031,21 @@ __global__ void k_seed_serve(SolveArgs a, const uint32_t* seeds, uint32_t n_seed
         const uint32_t old = atomicMax(&a.qwave[t], wave + 2u);
         if (old < wave + 2u) list_append(a, cur ^ 1u, t);
     }
-    if (s == 0) a.ctrl->done = 0u;
+}
+
+// Serving admission, last step: the loop leaves its idle state so the next wave boundary
+// reports the new slots (with or without seeds).
+__global__ void k_wake(GhcmCtrl* ctrl) {
+    if (threadIdx.x == 0) ctrl->done = 0u;
 }
 
 // Serving admission: the slot's instance entry written, its tile range reset and mapped
 // to the slot.
-__global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<GhcmInst*>(a.inst)[slot] = in;
+__global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t gen) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const_cast<GhcmInst*>(a.inst)[slot] = in;
+        const_cast<uint32_t*>(a.sgen)[slot] = gen;
+    }
     const uint32_t base = in.tile_base, count = in.tiles_x * in.tiles_y;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t t = base + i;
@@ -1268,12 +1372,44 @@ hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds, uint32_t
 }
 
 hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
-                              hipStream_t s) {
+                              uint32_t gen, hipStream_t s) {
     const uint32_t count = in.tiles_x * in.tiles_y;
     unsigned blocks = (count + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_admit_tiles, dim3(blocks), dim3(256), 0, s, a, in, slot);
+    hipLaunchKernelGGL(k_admit_tiles, dim3(blocks), dim3(256), 0, s, a, in, slot, gen);
+    return hipGetLastError();
+}
+
+hipError_t launch_wake(GhcmCtrl* ctrl, hipStream_t s) {
+    hipLaunchKernelGGL(k_wake, dim3(1), dim3(64), 0, s, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_t n_inst,
+                             uint64_t max_cells, uint32_t n_select, unsigned long long* totals,
+                             hipStream_t s) {
+    uint64_t bx = (max_cells + 256ull * 16 - 1) / (256ull * 16);
+    if (bx < 1) bx = 1;
+    if (bx > 1024) bx = 1024;
+    const uint32_t by = n_inst < 65535u ? (n_inst ? n_inst : 1u) : 65535u;
+    hipLaunchKernelGGL(k_solve_init, dim3((unsigned)bx, by), dim3(256), 0, s, a, h_inst, n_inst,
+                       n_select, totals);
+    return hipGetLastError();
+}
+
+hipError_t launch_publish_totals(const unsigned long long* totals, HostRec* rec, uint32_t tag,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_publish_totals, dim3(1), dim3(64), 0, s, totals, rec, tag);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    size_t blocks = (bytes + 16 * 256 * 4 - 1) / (16 * 256 * 4);
+    if (blocks < 1) blocks = 1;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, bytes);
     return hipGetLastError();
 }
 

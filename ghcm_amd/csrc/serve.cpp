@@ -1,19 +1,22 @@
 // serve.cpp -- continuous-batching solve server (ghcm_serve_*).
 //
 // One device wave loop runs for the lifetime of the server and never restarts between
-// requests.  A request is admitted at a wave boundary when an instance slot, a range of
-// tile state and room for its field and time buffer inside the HBM arena are all free
-// (first come, first served).  Admission is three launches on the loop's stream: the
-// slot's instance entry and tile range are reset (k_admit_tiles), the time buffer is set
-// to +INF, and the seeds queue their tiles for the next wave (k_seed_serve), exactly as
-// a plain solve's seeds are queued for its first wave (src/eikonal_hcm.cpp:189-238).
-// From then on the request's tiles compete in the same top-n selection as every other
-// live request's.  Each wave boundary counts the queued tiles per slot (k_wave_end);
-// the host reads those counts once per chunk of waves, and a request whose count is 0
-// at a boundary after its admission is finished: tiles are only ever queued by seeds or
-// by a solved neighbour of the same instance, so nothing can reopen it.  Its times are
-// copied out on a second stream, and its slot, tiles and arena bytes are freed once the
-// copy has landed.
+// requests.  A request is admitted between waves when an instance slot, a range of tile
+// state and (for host buffers) room for its field and time buffer inside the HBM arena
+// are all free, first come first served.  Admission is a few launches on the loop's
+// stream: the slot's instance entry, admission generation and tile range are reset
+// (k_admit_tiles), the time buffer is set to +INF, the seeds queue their tiles for the
+// next wave (k_seed_serve, as a plain solve's seeds are queued for its first wave,
+// src/eikonal_hcm.cpp:189-238) and the loop leaves its idle state (k_wake).  From then
+// on the request's tiles compete in the same top-n selection as every live request's.
+//
+// Every wave boundary (k_wave_end) writes, into mapped pinned host memory, each slot's
+// queued-tile count tagged with the slot's generation.  A request is finished at the
+// first boundary that reports its generation with a count of 0: tiles are only queued
+// by seeds or by a solved neighbour of the same instance, so nothing can reopen it.
+// Host-buffer requests are staged through pinned blocks and moved by copy kernels on two
+// side streams (upload before admission, download after completion), ordered against the
+// loop by events; the loop itself never waits on the runtime.
 //
 // The reference solves one field per call (src/eikonal_hcm.cpp:186-529); this loop is
 // the serving form of the same wave algorithm.
@@ -91,6 +94,71 @@ private:
     uint64_t used_ = 0;
 };
 
+// Pinned host blocks for staging request fields and results, reused across requests.
+// Sizes are rounded up to a quarter-power-of-two class so blocks fit later requests.
+class PinnedPool {
+public:
+    static size_t size_class(size_t b) {
+        size_t p = 1u << 16;
+        while (p < b && p * 2 <= b) p <<= 1;
+        if (p >= b) return p;
+        for (int q = 5; q <= 8; ++q)
+            if (p / 4 * q >= b) return p / 4 * q;
+        return p * 2;
+    }
+    // A block of at least `bytes`, mapped for kernels (*dev: its device address).
+    void* get(size_t bytes, void** dev) {
+        const size_t c = size_class(bytes);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.find(c);
+            if (it != free_.end() && !it->second.empty()) {
+                void* p = it->second.back();
+                it->second.pop_back();
+                *dev = dev_[p];
+                return p;
+            }
+        }
+        void* p = nullptr;
+        void* d = nullptr;
+        if (hipHostMalloc(&p, c, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return nullptr;
+        if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+            (void)hipHostFree(p);
+            return nullptr;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        all_.push_back(p);
+        dev_[p] = d;
+        bytes_ += c;
+        *dev = d;
+        return p;
+    }
+    void put(void* p, size_t bytes) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu_);
+        free_[size_class(bytes)].push_back(p);
+    }
+    void release() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (void* p : all_) (void)hipHostFree(p);
+        all_.clear();
+        free_.clear();
+        dev_.clear();
+        bytes_ = 0;
+    }
+    uint64_t bytes() {
+        std::lock_guard<std::mutex> lk(mu_);
+        return bytes_;
+    }
+
+private:
+    std::mutex mu_;
+    std::map<size_t, std::vector<void*>> free_;
+    std::vector<void*> all_;
+    std::unordered_map<void*, void*> dev_;
+    uint64_t bytes_ = 0;
+};
+
 constexpr uint64_t kAlign = 256;
 inline uint64_t align_up(uint64_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 
@@ -98,52 +166,56 @@ enum class ReqState { Pending, Live, Copying, Done };
 
 struct Request {
     uint64_t ticket = 0;
-    const void* field = nullptr;
+    bool device = false;                // device-resident request (no staging, no arena)
+    const void* d_field = nullptr;      // device request: caller's buffers
+    float* d_time = nullptr;
+    void* h_field = nullptr;            // host request: pinned staging of the field ...
+    void* h_field_dev = nullptr;
+    void* h_time = nullptr;             // ... and of the result
+    void* h_time_dev = nullptr;
     uint32_t w = 0, h = 0;
     std::vector<uint32_t> seeds;        // (x, y) pairs
-    float* time_out = nullptr;
+    float* time_out = nullptr;          // host request: caller's result buffer
+    bool waited = false;                // a ghcm_serve_wait is collecting it
     ReqState state = ReqState::Pending;
     int status = GHCM_OK;
     Clock::time_point t_submit, t_admit, t_done;
     // placement
-    uint32_t slot = 0;
+    uint32_t slot = 0, gen = 0;
     uint64_t arena_off = 0, arena_bytes = 0;
     uint64_t tile_base = 0, tiles = 0;
     uint32_t tiles_x = 0, tiles_y = 0;
-    uint64_t admit_seq = 0;             // first chunk enqueued after admission
+    uint32_t admit_tick = 0;            // tick of the first boundary after admission
     uint64_t waves = 0;
-    hipEvent_t ev_done = nullptr;       // result copy landed
-};
-
-struct SnapHead {                        // first words of GhcmCtrl read back per chunk
-    uint32_t wave, cur, done, n_select, n_jobs, jobs_from_list, busy_waves, total;
+    hipEvent_t ev_ready = nullptr;      // host request: the loop passed its completion
+    hipEvent_t ev_done = nullptr;       // host request: result copy landed
 };
 
 }  // namespace
 
 struct ghcm_server {
-    static constexpr int kSlots = 4;     // chunks in flight
-
     int ordinal = 0;
     ghcm_config cfg{};
     ghcm_serve_config scfg{};
     int kind = GHCM_FIELD_COST_F32;
-    uint32_t tw = 0, th = 0, n_sel = 0, grid = 1, chunk = 4, slots = 0, max_tiles = 0;
+    uint32_t tw = 0, th = 0, n_sel = 0, grid = 1, lead = 8, step = 1, slots = 0, max_tiles = 0;
     uint64_t budget = 0;
     solve_launcher_t launch = nullptr;
-    hipStream_t stream = nullptr, copy_stream = nullptr;
+    hipStream_t stream = nullptr, up_stream = nullptr, down_stream = nullptr;
     hipEvent_t ev_upload = nullptr;
-    hipEvent_t ev_slot[kSlots] = {};
-    uint64_t slot_seq[kSlots] = {};
     hipGraphExec_t gexec = nullptr;
-    DevBuf arena, inst, tinst, qwave, key, mask, tmin, list, tstat, sel, ctrl, iact;
-    DevBuf d_seeds[kSlots];
-    PinnedBuf h_seeds[kSlots], h_snap[kSlots];
+    DevBuf arena, inst, tinst, qwave, key, mask, tmin, list, tstat, sel, ctrl, sgen, totals;
+    static constexpr int kSeedRing = 8;   // seed staging buffers, reused round robin
+    PinnedBuf h_seeds[kSeedRing], rec;
+    hipEvent_t ev_seed[kSeedRing] = {};  // recorded after the launch that read each buffer
+    int seed_next = 0;
+    PinnedPool pool;
     SolveArgs a{};
     RangeAlloc arena_alloc, tile_alloc;
     std::vector<uint32_t> free_slots;
     std::vector<Request*> live;          // by slot (server thread only)
     std::vector<hipEvent_t> ev_pool;     // server thread only
+    uint32_t next_gen = 1;               // server thread only
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -159,9 +231,9 @@ struct ghcm_server {
     int init();
     void run();
     int loop();
-    int admit(std::vector<Request*>& batch, int head, uint64_t seq);
-    int enqueue_chunk(int head);
-    int harvest(int sl, std::vector<Request*>& finishing);
+    int admit(std::vector<Request*>& batch, uint32_t enqueued);
+    int enqueue_step();
+    int get_event(hipEvent_t* ev);
     bool place(Request* r);
     void unplace(Request* r);
     void release_all();
@@ -177,9 +249,9 @@ int ghcm_server::init() {
     HIP_TRY(hipGetDeviceProperties(&p, ordinal));
     const int cus = p.multiProcessorCount;
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&down_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming));
-    for (int k = 0; k < kSlots; ++k) HIP_TRY(hipEventCreateWithFlags(&ev_slot[k], hipEventDisableTiming));
 
     const SolveKernel sk = get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, false);
     launch = sk.launch;
@@ -190,7 +262,8 @@ int ghcm_server::init() {
                          : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * cus));
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, cus);
     grid = std::max(1u, std::min(n_sel, resident));
-    chunk = cfg.waves_per_poll ? cfg.waves_per_poll : 4;
+    lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);
+    step = cfg.launch_mode == 2 ? std::max<uint32_t>(1, lead / 2) : 1;
 
     const uint64_t T = max_tiles;
     const uint32_t shard_cap = (uint32_t)((T + kShards - 1) / kShards);
@@ -206,9 +279,13 @@ int ghcm_server::init() {
     if ((rc = tstat.reserve(12 * T))) return rc;
     if ((rc = sel.reserve(4ull * std::max<uint64_t>(1, std::min<uint64_t>(n_sel, T))))) return rc;
     if ((rc = ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
-    if ((rc = iact.reserve(4ull * slots))) return rc;
-    for (int k = 0; k < kSlots; ++k)
-        if ((rc = h_snap[k].reserve(sizeof(SnapHead) + 4ull * slots))) return rc;
+    if ((rc = sgen.reserve(4ull * slots))) return rc;
+    if ((rc = totals.reserve(3 * sizeof(unsigned long long)))) return rc;
+    if ((rc = rec.reserve(sizeof(HostRec)))) return rc;
+    for (int k = 0; k < kSeedRing; ++k) {
+        if ((rc = h_seeds[k].reserve(12 * 1024))) return rc;
+        HIP_TRY(hipEventCreateWithFlags(&ev_seed[k], hipEventDisableTiming));
+    }
 
     a.inst = inst.as<GhcmInst>();
     a.n_inst = slots;
@@ -225,29 +302,25 @@ int ghcm_server::init() {
     a.ctrl = ctrl.as<GhcmCtrl>();
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
-    a.iact = iact.as<uint32_t>();
-    a.n_iact = slots;
+    a.rec = rec.dev_as<HostRec>();
+    a.sgen = sgen.as<uint32_t>();
+    a.n_slots = slots;
 
-    // Idle loop: empty lists, `done` set, every slot's count 0.
-    if ((rc = h_snap[0].reserve(std::max<size_t>(sizeof(GhcmCtrl), sizeof(SnapHead) + 4ull * slots))))
-        return rc;
-    GhcmCtrl* h0 = h_snap[0].as<GhcmCtrl>();
-    std::memset(h0, 0, sizeof(GhcmCtrl));
-    h0->done = 1;
-    h0->n_select = n_sel;
-    HIP_TRY(hipMemcpyAsync(a.ctrl, h0, sizeof(GhcmCtrl), hipMemcpyHostToDevice, stream));
-    HIP_TRY(launch_fill_u32(a.iact, slots, 0u, stream));
+    // Empty loop: lists empty, every slot generation 0 (no request carries it).
+    std::memset(rec.p, 0, sizeof(HostRec));
+    HIP_TRY(launch_solve_init(a, nullptr, 0, 0, n_sel, totals.as<unsigned long long>(), stream));
+    HIP_TRY(launch_fill_u32(sgen.as<uint32_t>(), slots, 0u, stream));
     HIP_TRY(launch_fill_u32(a.qwave, T, 0u, stream));
     HIP_TRY(launch_fill_u32(const_cast<uint32_t*>(a.tinst), T, 0u, stream));
     HIP_TRY(launch_fill_u32(a.tstat, 3 * T, 0u, stream));
     HIP_TRY(hipStreamSynchronize(stream));
 
     if (cfg.launch_mode == 2) {
-        // One chunk of waves as a HIP graph: the arguments never change for a server.
+        // `step` waves as one HIP graph: the arguments never change for a server.
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         hipError_t ce = hipSuccess;
-        for (uint32_t k = 0; k < chunk && ce == hipSuccess; ++k) {
+        for (uint32_t k = 0; k < step && ce == hipSuccess; ++k) {
             ce = launch(a, grid, stream, nullptr, nullptr);
             if (ce == hipSuccess) ce = launch_wave_end(a, 0u, stream);
         }
@@ -272,48 +345,62 @@ int ghcm_server::init() {
     st.max_tiles = T;
     st.slots = slots;
     st.n_active = n_sel;
-    st.waves_per_chunk = chunk;
+    st.waves_per_chunk = step;
     return GHCM_OK;
 }
 
 void ghcm_server::release_all() {
-    if (stream) (void)hipStreamSynchronize(stream);
-    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-    for (DevBuf* b : {&arena, &inst, &tinst, &qwave, &key, &mask, &tmin, &list, &tstat, &sel, &ctrl, &iact})
+    for (hipStream_t s : {stream, up_stream, down_stream})
+        if (s) (void)hipStreamSynchronize(s);
+    for (DevBuf* b : {&arena, &inst, &tinst, &qwave, &key, &mask, &tmin, &list, &tstat, &sel, &ctrl,
+                      &sgen, &totals})
         b->release();
-    for (int k = 0; k < kSlots; ++k) {
-        d_seeds[k].release();
+    for (int k = 0; k < kSeedRing; ++k) {
         h_seeds[k].release();
-        h_snap[k].release();
-        if (ev_slot[k]) (void)hipEventDestroy(ev_slot[k]);
-        ev_slot[k] = nullptr;
+        if (ev_seed[k]) (void)hipEventDestroy(ev_seed[k]);
+        ev_seed[k] = nullptr;
     }
+    rec.release();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     ev_pool.clear();
+    pool.release();
     if (gexec) (void)hipGraphExecDestroy(gexec);
     gexec = nullptr;
     if (ev_upload) (void)hipEventDestroy(ev_upload);
     ev_upload = nullptr;
-    if (copy_stream) (void)hipStreamDestroy(copy_stream);
-    if (stream) (void)hipStreamDestroy(stream);
-    copy_stream = stream = nullptr;
+    for (hipStream_t* s : {&stream, &up_stream, &down_stream}) {
+        if (*s) (void)hipStreamDestroy(*s);
+        *s = nullptr;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
 // the server thread
 // ---------------------------------------------------------------------------------------
 
+int ghcm_server::get_event(hipEvent_t* ev) {
+    if (ev_pool.empty()) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev_pool.push_back(e);
+    }
+    *ev = ev_pool.back();
+    ev_pool.pop_back();
+    return GHCM_OK;
+}
+
 // Placement of one request (caller holds mu): slot, tile range and arena bytes, or
 // nothing at all.
 bool ghcm_server::place(Request* r) {
     if (free_slots.empty()) return false;
     const uint64_t cells = (uint64_t)r->w * r->h;
-    const uint64_t fb = align_up(cells * kind_bytes(kind)), tb = align_up(cells * 4);
+    const uint64_t fb = r->device ? 0 : align_up(cells * kind_bytes(kind));
+    const uint64_t tb = r->device ? 0 : align_up(cells * 4);
     uint64_t off = 0, tbase = 0;
     const uint64_t ntiles = (uint64_t)r->tiles_x * r->tiles_y;
-    if (!arena_alloc.alloc(fb + tb, &off)) return false;
+    if (fb + tb && !arena_alloc.alloc(fb + tb, &off)) return false;
     if (!tile_alloc.alloc(ntiles, &tbase)) {
-        arena_alloc.release(off, fb + tb);
+        if (fb + tb) arena_alloc.release(off, fb + tb);
         return false;
     }
     r->slot = free_slots.back();
@@ -327,122 +414,103 @@ bool ghcm_server::place(Request* r) {
 }
 
 void ghcm_server::unplace(Request* r) {
-    arena_alloc.release(r->arena_off, r->arena_bytes);
+    if (r->arena_bytes) arena_alloc.release(r->arena_off, r->arena_bytes);
     tile_alloc.release(r->tile_base, r->tiles);
     free_slots.push_back(r->slot);
     r->arena_bytes = r->tiles = 0;
 }
 
-// Admission of a batch of placed requests, enqueued before chunk `seq` (slot `head`).
-int ghcm_server::admit(std::vector<Request*>& batch, int head, uint64_t seq) {
+// Admission of a batch of placed requests before the wave with tick `enqueued` + 1.
+int ghcm_server::admit(std::vector<Request*>& batch, uint32_t enqueued) {
     char* base = arena.as<char>();
     size_t n_seeds = 0;
+    bool uploads = false;
     for (Request* r : batch) {
         const uint64_t cells = (uint64_t)r->w * r->h;
-        HIP_TRY(hipMemcpyAsync(base + r->arena_off, r->field, cells * kind_bytes(kind),
-                               hipMemcpyHostToDevice, copy_stream));
+        if (!r->device) {
+            HIP_TRY(launch_copy(r->h_field_dev, base + r->arena_off, cells * kind_bytes(kind), up_stream));
+            uploads = true;
+        }
         n_seeds += r->seeds.size() / 2;
     }
-    HIP_TRY(hipEventRecord(ev_upload, copy_stream));
-    HIP_TRY(hipStreamWaitEvent(stream, ev_upload, 0));
-    int rc;
-    if ((rc = h_seeds[head].reserve(12 * std::max<size_t>(1, n_seeds)))) return rc;
-    if ((rc = d_seeds[head].reserve(12 * std::max<size_t>(1, n_seeds)))) return rc;
-    uint32_t* hs = h_seeds[head].as<uint32_t>();
+    if (uploads) {
+        HIP_TRY(hipEventRecord(ev_upload, up_stream));
+        HIP_TRY(hipStreamWaitEvent(stream, ev_upload, 0));
+    }
+    // Seed staging: a ring of mapped buffers read by k_seed_serve; a buffer is reused once
+    // the launch that read it, kSeedRing admissions ago, has run.
+    const int ring = seed_next;
+    if (n_seeds) {
+        seed_next = (seed_next + 1) % kSeedRing;
+        HIP_TRY(ghcm_capi::spin_wait(ev_seed[ring]));
+        int rc = h_seeds[ring].reserve(12 * n_seeds);
+        if (rc) return rc;
+    }
+    uint32_t* hs = h_seeds[ring].as<uint32_t>();
     size_t k = 0;
     for (Request* r : batch) {
         const uint64_t cells = (uint64_t)r->w * r->h;
         GhcmInst g{};
-        g.field = base + r->arena_off;
-        g.time = reinterpret_cast<float*>(base + r->arena_off + align_up(cells * kind_bytes(kind)));
+        if (r->device) {
+            g.field = r->d_field;
+            g.time = r->d_time;
+        } else {
+            g.field = base + r->arena_off;
+            g.time = reinterpret_cast<float*>(base + r->arena_off + align_up(cells * kind_bytes(kind)));
+        }
         g.w = r->w;
         g.h = r->h;
         g.tiles_x = r->tiles_x;
         g.tiles_y = r->tiles_y;
         g.tile_base = (uint32_t)r->tile_base;
-        HIP_TRY(launch_admit_tiles(a, g, r->slot, stream));
+        r->gen = next_gen++;
+        if (next_gen == 0) next_gen = 1;
+        HIP_TRY(launch_admit_tiles(a, g, r->slot, r->gen, stream));
         HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(g.time), cells, kInfBits, stream));
         for (size_t s = 0; s + 1 < r->seeds.size(); s += 2) {
             hs[k++] = r->seeds[s];
             hs[k++] = r->seeds[s + 1];
             hs[k++] = r->slot;
         }
-        r->admit_seq = seq;
+        r->admit_tick = enqueued + 1;
         r->state = ReqState::Live;
         live[r->slot] = r;
     }
     if (n_seeds) {
-        HIP_TRY(hipMemcpyAsync(d_seeds[head].p, hs, 12 * n_seeds, hipMemcpyHostToDevice, stream));
-        HIP_TRY(launch_seed_serve(a, d_seeds[head].as<uint32_t>(), (uint32_t)n_seeds, tw, th, stream));
+        HIP_TRY(launch_seed_serve(a, h_seeds[ring].dev_as<uint32_t>(), (uint32_t)n_seeds, tw, th, stream));
+        HIP_TRY(hipEventRecord(ev_seed[ring], stream));
     }
+    HIP_TRY(launch_wake(a.ctrl, stream));
     return GHCM_OK;
 }
 
-int ghcm_server::enqueue_chunk(int head) {
+int ghcm_server::enqueue_step() {
     if (gexec) {
         HIP_TRY(hipGraphLaunch(gexec, stream));
     } else {
-        for (uint32_t k = 0; k < chunk; ++k) {
-            HIP_TRY(launch(a, grid, stream, nullptr, nullptr));
-            HIP_TRY(launch_wave_end(a, 0u, stream));
-        }
-    }
-    char* snap = h_snap[head].as<char>();
-    HIP_TRY(hipMemcpyAsync(snap, a.ctrl, sizeof(SnapHead), hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipMemcpyAsync(snap + sizeof(SnapHead), a.iact, 4ull * slots, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipEventRecord(ev_slot[head], stream));
-    return GHCM_OK;
-}
-
-// Reads the snapshot of a completed chunk: requests with no queued tile are finished.
-int ghcm_server::harvest(int sl, std::vector<Request*>& finishing) {
-    const char* snap = h_snap[sl].as<char>();
-    SnapHead hd;
-    std::memcpy(&hd, snap, sizeof hd);
-    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(snap + sizeof(SnapHead));
-    const uint64_t seq = slot_seq[sl];
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        st.busy_waves = hd.busy_waves;
-        st.loop_wave = hd.wave;
-    }
-    char* base = arena.as<char>();
-    for (uint32_t s = 0; s < slots; ++s) {
-        Request* r = live[s];
-        if (!r || r->admit_seq > seq || cnt[s] != 0u) continue;
-        live[s] = nullptr;
-        r->waves = (seq - r->admit_seq + 1) * chunk;
-        if (ev_pool.empty()) {
-            hipEvent_t e;
-            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            ev_pool.push_back(e);
-        }
-        r->ev_done = ev_pool.back();
-        ev_pool.pop_back();
-        const uint64_t cells = (uint64_t)r->w * r->h;
-        const char* tsrc = base + r->arena_off + align_up(cells * kind_bytes(kind));
-        HIP_TRY(hipMemcpyAsync(r->time_out, tsrc, cells * 4, hipMemcpyDeviceToHost, copy_stream));
-        HIP_TRY(hipEventRecord(r->ev_done, copy_stream));
-        r->state = ReqState::Copying;
-        finishing.push_back(r);
+        HIP_TRY(launch(a, grid, stream, nullptr, nullptr));
+        HIP_TRY(launch_wave_end(a, 0u, stream));
     }
     return GHCM_OK;
 }
 
 int ghcm_server::loop() {
     HIP_TRY(hipSetDevice(ordinal));
+    volatile HostRec* hr = rec.as<HostRec>();
     std::vector<Request*> finishing;
-    uint64_t seq = 0;
-    int inflight = 0, head = 0;
+    uint32_t enqueued = 0;               // wave boundaries launched (modulo 2^32)
+    uint32_t scanned_tick = 0;
     uint32_t n_live = 0;
+    uint64_t idle_spins = 0;
+    char* base = arena.as<char>();
     for (;;) {
+        bool progressed = false;
         std::vector<Request*> batch;
         {
             std::unique_lock<std::mutex> lk(mu);
             for (;;) {
                 if (stop && !drain) return GHCM_OK;
-                const bool busy = n_live > 0 || !finishing.empty() || inflight > 0;
-                if (busy || !pending.empty()) break;
+                if (n_live > 0 || !finishing.empty() || !pending.empty()) break;
                 if (stop) return GHCM_OK;           // drained
                 cv_work.wait(lk);
             }
@@ -455,57 +523,80 @@ int ghcm_server::loop() {
         if (!batch.empty()) {
             const auto now = Clock::now();
             for (Request* r : batch) r->t_admit = now;
-            int rc = admit(batch, head, seq);
+            int rc = admit(batch, enqueued);
             if (rc) return rc;
             n_live += (uint32_t)batch.size();
+            progressed = true;
             std::lock_guard<std::mutex> lk(mu);
             st.peak_live = std::max<uint64_t>(st.peak_live, n_live);
         }
-        if (n_live > 0) {
-            int rc = enqueue_chunk(head);
+        const uint32_t ticks = hr->ticks;
+        // Keep `lead` waves queued ahead of the last executed boundary while work is live.
+        if (n_live > 0 && (uint32_t)(enqueued - ticks) < lead) {
+            int rc = enqueue_step();
             if (rc) return rc;
-            slot_seq[head] = seq++;
-            ++inflight;
-            head = (head + 1) % kSlots;
+            enqueued += step;
+            progressed = true;
             std::lock_guard<std::mutex> lk(mu);
-            st.waves_enqueued += chunk;
+            st.waves_enqueued += step;
         }
-        // Harvest completed chunks: always the oldest when the ring is full or nothing
-        // more will be launched, otherwise only those already finished.
-        while (inflight > 0) {
-            const int oldest = (head - inflight + kSlots) % kSlots;
-            if (inflight == kSlots || n_live == 0) {
-                HIP_TRY(hipEventSynchronize(ev_slot[oldest]));
-            } else {
-                const hipError_t q = hipEventQuery(ev_slot[oldest]);
-                if (q == hipErrorNotReady) break;
-                if (q != hipSuccess) return fail(GHCM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        // Completions: each boundary reports (generation, queued tiles) per slot.
+        if (n_live > 0 && ticks != scanned_tick) {
+            scanned_tick = ticks;
+            std::vector<Request*> done_now;
+            for (uint32_t s = 0; s < slots; ++s) {
+                Request* r = live[s];
+                if (!r) continue;
+                const unsigned long long v = hr->slot[s];
+                if ((uint32_t)(v >> 32) != r->gen || (uint32_t)v != 0u) continue;
+                live[s] = nullptr;
+                --n_live;
+                r->waves = (uint32_t)(ticks - r->admit_tick) + 1u;
+                r->state = ReqState::Copying;
+                if (!r->device) {
+                    // Times are final; the copy runs behind the waves already queued.
+                    int rc = get_event(&r->ev_ready);
+                    if (!rc) rc = get_event(&r->ev_done);
+                    if (rc) return rc;
+                    const uint64_t cells = (uint64_t)r->w * r->h;
+                    const char* tsrc = base + r->arena_off + align_up(cells * kind_bytes(kind));
+                    HIP_TRY(hipEventRecord(r->ev_ready, stream));
+                    HIP_TRY(hipStreamWaitEvent(down_stream, r->ev_ready, 0));
+                    HIP_TRY(launch_copy(tsrc, r->h_time_dev, cells * 4, down_stream));
+                    HIP_TRY(hipEventRecord(r->ev_done, down_stream));
+                }
+                finishing.push_back(r);
+                progressed = true;
             }
-            const size_t before = finishing.size();
-            int rc = harvest(oldest, finishing);
-            if (rc) return rc;
-            n_live -= (uint32_t)(finishing.size() - before);
-            --inflight;
+            std::lock_guard<std::mutex> lk(mu);
+            st.busy_waves = hr->busy_waves;
+            st.loop_wave = hr->wave;
         }
-        // Results whose copy has landed are handed back and their resources freed.
+        // Requests whose results are in place are handed back; their resources freed.
         if (!finishing.empty()) {
-            if (n_live == 0 && inflight == 0) HIP_TRY(hipEventSynchronize(finishing.front()->ev_done));
-            std::vector<Request*> keep;
-            std::vector<Request*> done;
+            std::vector<Request*> keep, done;
             for (Request* r : finishing) {
-                const hipError_t q = hipEventQuery(r->ev_done);
-                if (q == hipErrorNotReady) { keep.push_back(r); continue; }
-                if (q != hipSuccess) return fail(GHCM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
-                ev_pool.push_back(r->ev_done);
-                r->ev_done = nullptr;
+                if (r->ev_done) {
+                    const hipError_t q = hipEventQuery(r->ev_done);
+                    if (q == hipErrorNotReady) { keep.push_back(r); continue; }
+                    if (q != hipSuccess) return fail(GHCM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+                    ev_pool.push_back(r->ev_ready);
+                    ev_pool.push_back(r->ev_done);
+                    r->ev_ready = r->ev_done = nullptr;
+                }
                 done.push_back(r);
             }
             finishing.swap(keep);
             if (!done.empty()) {
+                progressed = true;
                 const auto now = Clock::now();
                 std::lock_guard<std::mutex> lk(mu);
                 for (Request* r : done) {
                     unplace(r);
+                    if (r->h_field) {
+                        pool.put(r->h_field, (size_t)r->w * r->h * kind_bytes(kind));
+                        r->h_field = nullptr;
+                    }
                     r->t_done = now;
                     r->status = GHCM_OK;
                     r->state = ReqState::Done;
@@ -514,14 +605,24 @@ int ghcm_server::loop() {
                 cv_done.notify_all();
             }
         }
+        if (progressed) {
+            idle_spins = 0;
+        } else if (++idle_spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(stream);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail(GHCM_ERR_HIP, "serving loop: %s", hipGetErrorString(q));
+            std::this_thread::yield();
+        } else {
+            __builtin_ia32_pause();
+        }
     }
 }
 
 void ghcm_server::run() {
     const int rc = loop();
     // No copy may still be writing into a caller's buffer once its request is reported.
-    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
-    if (stream) (void)hipStreamSynchronize(stream);
+    for (hipStream_t s : {down_stream, up_stream, stream})
+        if (s) (void)hipStreamSynchronize(s);
     std::lock_guard<std::mutex> lk(mu);
     if (rc != GHCM_OK) {
         fatal = rc;
@@ -607,11 +708,11 @@ extern "C" int ghcm_serve_start(int ordinal, const ghcm_config* cfg, const ghcm_
     return GHCM_OK;
 }
 
-extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, uint32_t h,
-                                 const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
-                                 uint64_t* ticket) {
-    if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
-    if (!field || !time_out || !ticket) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+namespace {
+
+// Checks shared by both submit forms; fills the request's size and tile counts.
+int check_request(ghcm_server* s, uint32_t w, uint32_t h, const uint32_t* seeds_xy,
+                  uint32_t n_seeds, bool device, Request* r) {
     if (w == 0 || h == 0) return fail(GHCM_ERR_INVALID_ARG, "zero-sized field");
     if (n_seeds && !seeds_xy) return fail(GHCM_ERR_INVALID_ARG, "seeds are null");
     for (uint32_t k = 0; k < n_seeds; ++k)
@@ -620,26 +721,28 @@ extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, 
                         seeds_xy[2 * k + 1], w, h);
     const uint64_t cells = (uint64_t)w * h;
     const uint32_t tx = (w + s->tw - 1) / s->tw, ty = (h + s->th - 1) / s->th;
-    const uint64_t bytes = align_up(cells * kind_bytes(s->kind)) + align_up(cells * 4);
+    const uint64_t bytes = device ? 0 : align_up(cells * kind_bytes(s->kind)) + align_up(cells * 4);
     if (bytes > s->budget)
         return fail(GHCM_ERR_OVER_BUDGET, "request needs %llu bytes, budget %llu",
                     (unsigned long long)bytes, (unsigned long long)s->budget);
     if ((uint64_t)tx * ty > s->max_tiles)
         return fail(GHCM_ERR_OVER_BUDGET, "request needs %llu tiles, capacity %u",
                     (unsigned long long)tx * ty, s->max_tiles);
-    Request* r = new (std::nothrow) Request();
-    if (!r) return fail(GHCM_ERR_OUT_OF_MEMORY, "request allocation failed");
-    r->field = field;
+    r->device = device;
     r->w = w;
     r->h = h;
     r->tiles_x = tx;
     r->tiles_y = ty;
-    r->time_out = time_out;
     r->seeds.assign(seeds_xy, seeds_xy + 2ull * n_seeds);
-    r->t_submit = Clock::now();
+    return GHCM_OK;
+}
+
+int enqueue_request(ghcm_server* s, Request* r, uint64_t* ticket) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop || s->exited) {
+            s->pool.put(r->h_field, (size_t)r->w * r->h * kind_bytes(s->kind));
+            s->pool.put(r->h_time, (size_t)r->w * r->h * 4);
             delete r;
             return s->fatal ? fail(s->fatal, "server failed: %s", s->fatal_msg.c_str())
                             : fail(GHCM_ERR_STOPPED, "server is stopping");
@@ -654,6 +757,55 @@ extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, 
     return GHCM_OK;
 }
 
+}  // namespace
+
+extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, uint32_t h,
+                                 const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
+                                 uint64_t* ticket) {
+    if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
+    if (!field || !time_out || !ticket) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    Request* r = new (std::nothrow) Request();
+    if (!r) return fail(GHCM_ERR_OUT_OF_MEMORY, "request allocation failed");
+    int rc = check_request(s, w, h, seeds_xy, n_seeds, false, r);
+    if (rc) {
+        delete r;
+        return rc;
+    }
+    r->t_submit = Clock::now();
+    // The field is staged in pinned memory here, on the caller's thread, so the server
+    // thread only issues asynchronous copies.
+    const size_t fbytes = (size_t)w * h * kind_bytes(s->kind), tbytes = (size_t)w * h * 4;
+    r->h_field = s->pool.get(fbytes, &r->h_field_dev);
+    r->h_time = s->pool.get(tbytes, &r->h_time_dev);
+    if (!r->h_field || !r->h_time) {
+        s->pool.put(r->h_field, fbytes);
+        s->pool.put(r->h_time, tbytes);
+        delete r;
+        return fail(GHCM_ERR_OUT_OF_MEMORY, "pinned staging of %zu bytes failed", fbytes + tbytes);
+    }
+    std::memcpy(r->h_field, field, fbytes);
+    r->time_out = time_out;
+    return enqueue_request(s, r, ticket);
+}
+
+extern "C" int ghcm_serve_submit_device(ghcm_server* s, const void* d_field, uint32_t w,
+                                        uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
+                                        float* d_time, uint64_t* ticket) {
+    if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
+    if (!d_field || !d_time || !ticket) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    Request* r = new (std::nothrow) Request();
+    if (!r) return fail(GHCM_ERR_OUT_OF_MEMORY, "request allocation failed");
+    int rc = check_request(s, w, h, seeds_xy, n_seeds, true, r);
+    if (rc) {
+        delete r;
+        return rc;
+    }
+    r->t_submit = Clock::now();
+    r->d_field = d_field;
+    r->d_time = d_time;
+    return enqueue_request(s, r, ticket);
+}
+
 extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s,
                                ghcm_serve_result* out) {
     if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
@@ -662,13 +814,24 @@ extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s
     if (it == s->tickets.end())
         return fail(GHCM_ERR_NOT_FOUND, "ticket %llu unknown or already collected", (unsigned long long)ticket);
     Request* r = it->second;
+    if (r->waited)
+        return fail(GHCM_ERR_INVALID_ARG, "ticket %llu is already being waited on", (unsigned long long)ticket);
+    r->waited = true;
     auto ready = [r] { return r->state == ReqState::Done; };
     if (timeout_s < 0) {
         s->cv_done.wait(lk, ready);
     } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+        r->waited = false;
         return fail(GHCM_ERR_TIMEOUT, "ticket %llu still running", (unsigned long long)ticket);
     }
+    s->tickets.erase(ticket);
+    const std::string fatal_msg = s->fatal_msg;
+    lk.unlock();
     const int status = r->status;
+    // Host request: the result lands in the caller's buffer from the pinned staging.
+    if (!r->device && status == GHCM_OK)
+        std::memcpy(r->time_out, r->h_time, (size_t)r->w * r->h * 4);
+    const auto t_out = Clock::now();
     if (out) {
         std::memset(out, 0, sizeof *out);
         out->status = status;
@@ -677,12 +840,15 @@ extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s
         out->waves = r->waves;
         const bool admitted = r->t_admit.time_since_epoch().count() != 0;
         out->queue_us = admitted ? std::chrono::duration<double, std::micro>(r->t_admit - r->t_submit).count() : 0.0;
-        out->latency_us = std::chrono::duration<double, std::micro>(r->t_done - r->t_submit).count();
+        out->latency_us = std::chrono::duration<double, std::micro>(
+            (r->device || status != GHCM_OK ? r->t_done : t_out) - r->t_submit).count();
     }
-    s->tickets.erase(it);
+    s->pool.put(r->h_field, (size_t)r->w * r->h * kind_bytes(s->kind));
+    s->pool.put(r->h_time, (size_t)r->w * r->h * 4);
     delete r;
-    if (status != GHCM_OK) return fail(status, "request %llu: %s", (unsigned long long)ticket,
-                                       s->fatal ? s->fatal_msg.c_str() : ghcm_status_string(status));
+    if (status != GHCM_OK)
+        return fail(status, "request %llu: %s", (unsigned long long)ticket,
+                    fatal_msg.empty() ? ghcm_status_string(status) : fatal_msg.c_str());
     return GHCM_OK;
 }
 

@@ -255,15 +255,22 @@ void ghcm_serve_config_default(ghcm_serve_config* scfg);
  * of waves as one HIP graph, 1 behaves as 0).  Allocates the arena and tile state. */
 int ghcm_serve_start(int ordinal, const ghcm_config* cfg, const ghcm_serve_config* scfg,
                      ghcm_server** out);
-/* Queues one solve.  `field` (W*H elements of the server's field kind) is read when the
- * request is admitted and must stay valid until it completes; the seeds are copied
- * here; `time_out` (W*H floats, host memory) receives the times.  GHCM_ERR_OVER_BUDGET
- * if the request can never fit. */
+/* Queues one solve.  `field` (W*H elements of the server's field kind) and the seeds
+ * are copied before this returns (the field into pinned staging); `time_out` (W*H
+ * floats, host memory) receives the times when the ticket is collected by
+ * ghcm_serve_wait.  GHCM_ERR_OVER_BUDGET if the request can never fit. */
 int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, uint32_t h,
                       const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
                       uint64_t* ticket);
+/* Device-resident request: `d_field` and `d_time` are device buffers on the server's
+ * device; the times are written into `d_time` in place and no arena bytes are used.
+ * Both buffers must stay valid until the request completes. */
+int ghcm_serve_submit_device(ghcm_server* s, const void* d_field, uint32_t w, uint32_t h,
+                             const uint32_t* seeds_xy, uint32_t n_seeds, float* d_time,
+                             uint64_t* ticket);
 /* Waits up to timeout_s seconds (< 0: no limit).  Returns the request's status (and
- * collects the ticket) or GHCM_ERR_TIMEOUT (the ticket stays valid). */
+ * collects the ticket) or GHCM_ERR_TIMEOUT (the ticket stays valid).  One waiter per
+ * ticket.  A host request's times are copied into its `time_out` by this call. */
 int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s, ghcm_serve_result* out);
 int ghcm_serve_get_stats(ghcm_server* s, ghcm_serve_stats* out);
 /* drain != 0: every submitted request completes first; 0: requests not yet complete

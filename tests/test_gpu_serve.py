@@ -199,3 +199,55 @@ def test_stop_without_drain_returns():
     srv.stop(drain=False)
     with pytest.raises(g.GhcmError):
         srv.submit(r.cost, r.seeds)
+
+
+def test_device_and_host_requests_mixed(mix):
+    """Device-resident requests (results in place) interleaved with host-buffer ones."""
+    print("case serve device+host mixed", flush=True)
+    with g.Context(0) as ctx, g.Server(0, **SWEEP) as srv:
+        dev = [(ctx.to_device(r.cost), ctx.alloc((r.h, r.w), np.float32)) for r in mix]
+        tickets = []
+        for i, r in enumerate(mix):
+            if i % 2:
+                tickets.append(srv.submit(r.cost, r.seeds))
+            else:
+                dc, dt = dev[i]
+                tickets.append(srv.submit_device(dc.ptr, r.w, r.h, r.seeds, dt.ptr))
+        res = [srv.wait(t, 60.0) for t in tickets]
+        st = srv.stats()
+        assert st["bytes_in_use"] == 0
+        for i, r in enumerate(mix):
+            t = res[i][0] if i % 2 else dev[i][1].download()
+            if i % 2 == 0:
+                assert res[i][0] is None
+            ref = oracle.fmm_f32(r.cost, r.seeds)
+            fin = np.isfinite(ref)
+            assert np.array_equal(np.isfinite(t), fin)
+            assert np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+        for dc, dt in dev:
+            dc.free()
+            dt.free()
+
+
+def test_second_waiter_on_same_ticket_is_refused():
+    print("case serve double wait", flush=True)
+    r = workloads._make(1024, 1024, "ring", 61, 1, 62)
+    with g.Server(0, **SWEEP) as srv:
+        t = srv.submit(r.cost, r.seeds)
+        got = {}
+
+        def waiter(name):
+            try:
+                got[name] = srv.wait(t, 60.0)
+            except g.GhcmError as e:
+                got[name] = e.status
+
+        ths = [threading.Thread(target=waiter, args=(k,)) for k in ("a", "b")]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+    ok = [k for k, v in got.items() if isinstance(v, tuple)]
+    assert len(ok) == 1, got
+    other = got["b" if ok[0] == "a" else "a"]
+    assert other in (g.ERR_NOT_FOUND, g.ERR_INVALID_ARG)
