@@ -11,9 +11,18 @@ in HBM before the timed region starts.
 Prints ONE JSON line on stdout: Mcells/s of the whole solve, the roofline entry of the
 dominant kernel (the tile-solve kernel: algorithmic bytes per launch / average launch
 duration, from HIP events stamped by each dispatch on the solver stream), and the CPU
-baseline (the oracle's FP32 FMM on this host).  Stamping every launch perturbs the loop
-(about 15 %), so `value` comes from the un-instrumented timed steps and the roofline from
-the same number of instrumented steps run right after them.
+baseline (the oracle's FP32 FMM on this host, one core; the FP32 FIM beside it).  Stamping
+every launch perturbs the loop, so `value` comes from the un-instrumented timed steps and
+the roofline from the same number of instrumented steps run right after them.
+
+Beside the headline the line carries:
+  batched    512 uneven instances in one device wave loop (autotuned over a small space)
+             vs the same solves launched one at a time, and the oracle FMM on all cores;
+  serving    the continuous-batching server on requests resident in HBM and on host
+             buffers: saturated solves/s vs one-at-a-time, p50/p99 latency under Poisson
+             arrivals at 80 % of the saturated rate;
+  precision  the 4096^2 field stored as FP32 cost and FP32/FP16/BF16 speed, L_inf/L1 vs
+             the FP64 FMM.
 """
 from __future__ import annotations
 
@@ -85,7 +94,17 @@ BATCH_CANDIDATES = [  # autotuner search space for the batch (kept small so the 
     dict(tile_w=64, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
     dict(tile_w=32, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
     dict(tile_w=16, tile_h=16, kernel=0, n_active=65536, inner_iter_cap=0, launch_mode=2),
+    dict(tile_w=32, tile_h=8, kernel=0, n_active=65536, inner_iter_cap=0, launch_mode=2),
+    dict(tile_w=8, tile_h=8, kernel=0, n_active=65536, inner_iter_cap=0, launch_mode=2),
 ]
+
+
+def host_cores() -> int:
+    """Cores this process may use (the GPU box grants 16 per GPU)."""
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except OSError:
+        return 1
 
 
 def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
@@ -127,9 +146,23 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
         for dc, dt in dev:
             dc.free()
             dt.free()
+    # CPU: the oracle FMM over independent instances on every host core, on a sample.
+    import concurrent.futures as cf
+
+    import oracle
+    cores = host_cores()
+    sample = insts[:64]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(lambda i: oracle.fmm_f32(i.cost, i.seeds), sample))
+    t_cpu = time.perf_counter() - t0
     keys = ("tile_w", "tile_h", "kernel", "n_active", "inner_iter_cap", "launch_mode")
     return {"workload": f"{len(insts)} uneven instances, sides 64..2048, {cells / 1e6:.0f} Mcells, "
                         "ring or smooth, 1-8 sources (seed 4040)",
+            "cpu_baseline": {"value": round(len(sample) / t_cpu, 2), "unit": "solves/s",
+                             "cores": cores, "kind": "port",
+                             "sample": f"oracle FP32 FMM over the first {len(sample)} batch "
+                                       f"instances, {cores} threads, {t_cpu:.1f} s"},
             "solves_per_s": round(len(insts) / t_batch, 1),
             "plain_solves_per_s": round(len(insts) / t_plain, 1),
             "speedup_vs_plain": round(t_plain / t_batch, 2),
@@ -143,9 +176,9 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             "max_rel_diff_vs_plain": worst}
 
 
-def _serve_runs(srv, submit, n_requests: int, seed: int) -> dict:
-    """Saturated run (every request submitted at once), then Poisson arrivals at half the
-    saturated rate; latencies are submit -> result, measured by the library."""
+def _serve_runs(srv, submit, n_requests: int, seed: int, load: float = 0.8) -> dict:
+    """Saturated run (every request submitted at once), then Poisson arrivals at `load`
+    times the saturated rate; latencies are submit -> result, measured by the library."""
     import threading
 
     import numpy as np
@@ -155,7 +188,7 @@ def _serve_runs(srv, submit, n_requests: int, seed: int) -> dict:
         results = [srv.wait(t) for t in tickets]
         t_sat = time.perf_counter() - t0
     sat_rate = n_requests / t_sat
-    rate = 0.5 * sat_rate
+    rate = load * sat_rate
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, n_requests)
     arrivals = np.cumsum(gaps) - gaps[0]
     tickets = [None] * n_requests
@@ -190,7 +223,8 @@ def _serve_runs(srv, submit, n_requests: int, seed: int) -> dict:
             "peak_live": srv.stats()["peak_live"]}
 
 
-def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256) -> dict:
+def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256,
+                  n_host_requests: int = 128) -> dict:
     """Continuous batching on requests already in HBM (device buffers) and on host buffers
     (PCIe in and out), each against the same solves made one at a time."""
     import numpy as np
@@ -204,8 +238,10 @@ def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 25
             return float("inf")
         return float(np.max(np.abs(t[fin] - p[fin])) / p[fin].max()) if fin.any() else 0.0
 
-    out = {"workload": f"{n_requests} requests cycling {n_distinct} square fields, sides "
-                       "log-uniform 128..2048, ring or smooth (sigma 8), 1-4 sources (seed 5050)"}
+    out = {"workload": f"{n_requests} requests ({n_host_requests} with host buffers) cycling "
+                       f"{n_distinct} square fields, sides log-uniform 128..4096, ring or smooth "
+                       "(sigma 32), 1-4 sources (seed 5050); Poisson arrivals at 80% of the "
+                       "saturated rate"}
     with g.Context(0, **serve_cfg) as ctx:
         # ---- device-resident requests ----
         d_fields = [ctx.to_device(r.cost) for r in insts]
@@ -247,12 +283,31 @@ def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 25
         for i in range(4):
             srv.wait(srv.submit(costs[i], insts[i].seeds))
         host = _serve_runs(srv, lambda i: srv.submit(costs[pick[i]], insts[pick[i]].seeds),
-                           n_requests, 5052)
+                           n_host_requests, 5052)
     worst = max(rel_diff(t, plain[pick[i]]) for i, (t, _) in enumerate(host.pop("results")))
     host.update({"plain_solves_per_s": round(1.0 / t_plain_host, 1),
                  "speedup_vs_plain": round(host["saturated_solves_per_s"] * t_plain_host, 2),
                  "max_rel_diff_vs_plain": worst})
     out["host_buffers"] = host
+    return out
+
+
+def bench_precision(g, cfg: dict, speed, seeds, ref64) -> dict:
+    """The 4096^2 field stored as FP32 cost, FP32/FP16/BF16 speed (times FP32) vs the FP64 FMM."""
+    import numpy as np
+
+    import oracle
+    out = {"reference": "oracle FP64 FMM", "tolerance_rel": 1e-4}
+    fields = [("cost_f32", g.cost_from_speed(speed), g.FIELD_COST_F32),
+              ("speed_f32", speed.astype(np.float32), g.FIELD_SPEED_F32),
+              ("speed_f16", speed.astype(np.float16), g.FIELD_SPEED_F16),
+              ("speed_bf16", g.to_bf16(speed.astype(np.float32)), g.FIELD_SPEED_BF16)]
+    with g.Context(0, **cfg) as ctx:
+        for name, field, kind in fields:
+            t, _ = ctx.solve(field, seeds, kind=kind)
+            m = oracle.error_metrics(t, ref64)
+            out[name] = {"linf": m["linf"], "l1": m["l1"], "linf_rel": m["linf"] / m["max_ref"],
+                         "finiteness_mismatch": m["finiteness_mismatch"]}
     return out
 
 
@@ -267,7 +322,8 @@ def main() -> int:
     ap.add_argument("--inner-cap", type=int, default=1, help="in-tile rounds per visit (0 = converge)")
     ap.add_argument("--waves-per-poll", type=int, default=8)
     ap.add_argument("--launch-mode", type=int, default=0, help="0 per-wave launches, 1 persistent")
-    ap.add_argument("--cpu-trials", type=int, default=3)
+    ap.add_argument("--cpu-trials", type=int, default=5)
+    ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision report")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
@@ -385,14 +441,30 @@ def main() -> int:
             f"{sv['host_buffers']['speedup_vs_plain']}x plain")
 
     if not args.no_cpu_baseline:
-        log(f"bench: CPU baseline ({args.cpu_trials} trials of the oracle FMM)")
+        log(f"bench: CPU baseline ({args.cpu_trials} trials of the oracle FMM, 1 of the FIM)")
         ref, cb = cpu_baseline(cost, seeds, args.cpu_trials)
         out["cpu_baseline"] = cb
+        import oracle
+        t0 = time.perf_counter()
+        ref_fim, _ = oracle.fim_f32(cost, seeds, 1e-5)
+        t_fim = time.perf_counter() - t0
+        out["cpu_baseline_fim"] = {"value": round(W * H / t_fim / 1e6, 3), "unit": "Mcells/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"full {W}x{H} workload, oracle FP32 FIM "
+                                             f"(minimum change 1e-5), 1 trial, {t_fim:.1f} s"}
         err = float(np.max(np.abs(t_gpu.astype(np.float64) - ref)))
+        fin = np.isfinite(ref_fim)
+        err_fim = float(np.max(np.abs(t_gpu[fin].astype(np.float64) - ref_fim[fin])))
         out["parity"] = {"linf_vs_cpu_fmm": err, "linf_rel": err / float(ref.max()),
+                         "linf_vs_cpu_fim": err_fim, "linf_rel_fim": err_fim / float(ref.max()),
                          "tolerance_rel": 1e-4}
     else:
         out["cpu_baseline"] = None
+    if not args.no_precision:
+        import oracle
+        log("bench: storage precision vs the FP64 FMM")
+        ref64 = oracle.fmm_f64(1.0 / speed, seeds)
+        out["precision"] = bench_precision(g, plain_cfg, speed, seeds, ref64)
     ctx.close()
     print(json.dumps(out), flush=True)
     return 0

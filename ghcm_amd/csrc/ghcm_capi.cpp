@@ -433,11 +433,14 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         const bool use_graph = cfg.launch_mode == 2 && !prof;
         const uint32_t step = use_graph ? std::max<uint32_t>(1, lead / 2) : 1;
         if (use_graph) {
-            // key: the kernel arguments and launch shape of one captured step of waves
-            std::vector<unsigned char> key(sizeof(SolveArgs) + 2 * sizeof(uint32_t));
-            std::memcpy(key.data(), &a, sizeof(SolveArgs));
-            std::memcpy(key.data() + sizeof(SolveArgs), &grid, sizeof(uint32_t));
-            std::memcpy(key.data() + sizeof(SolveArgs) + sizeof(uint32_t), &step, sizeof(uint32_t));
+            // key: the kernel (tile shape, field kind and style are one launcher each), its
+            // arguments and the launch shape of one captured step of waves
+            std::vector<unsigned char> key(sizeof(SolveArgs) + 2 * sizeof(uint32_t) + sizeof(launch));
+            unsigned char* kp = key.data();
+            std::memcpy(kp, &a, sizeof(SolveArgs));
+            std::memcpy(kp + sizeof(SolveArgs), &grid, sizeof(uint32_t));
+            std::memcpy(kp + sizeof(SolveArgs) + sizeof(uint32_t), &step, sizeof(uint32_t));
+            std::memcpy(kp + sizeof(SolveArgs) + 2 * sizeof(uint32_t), &launch, sizeof(launch));
             if (!ctx->gexec || key != ctx->gkey) {
                 if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
                 ctx->gexec = nullptr;

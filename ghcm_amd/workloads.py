@@ -5,9 +5,9 @@ Rings (256^2)    seven ring walls (speed 1e-3) with one gap each, centre source.
 Contrast (4096^2) the ring geometry scaled x16, walls at speed 1e-4 on background 1.
 Batch            `n` instances, W and H drawn independently from {64, 128, ..., 2048},
                  1-8 sources, ring (rings scaled to min(W, H)/256, walls 1e-3) or smooth
-                 (sigma 8) field chosen per instance; everything from one seed.
-Serving          square sizes log-uniform in [128, 2048] (rounded to 64), 1-4 sources,
-                 ring or smooth (sigma 8); Poisson arrival times at a given rate.
+                 (sigma 32, as the 4096^2 field) chosen per instance; one seed for all.
+Serving          square sizes log-uniform in [128, 4096] (rounded to 64), 1-4 sources,
+                 ring or smooth (sigma 32); Poisson arrival times at a given rate.
 
 The per-instance streams use splitmix64 (the library's generator) through
 ``gen_points``-style draws so the same seed gives the same batch on any host.
@@ -86,12 +86,12 @@ def batch(n: int = 512, seed: int = 4040, max_side: int = 2048, workers: int = 8
         h = int(sides[min(int(u[i, 1] * len(sides)), len(sides) - 1)])
         n_src = 1 + min(int(u[i, 2] * 8), 7)
         kind = "ring" if u[i, 3] < 0.5 else "smooth"
-        specs.append((w, h, kind, seed * 1000 + 2 * i, n_src, seed * 1000 + 2 * i + 1, 8.0, True))
+        specs.append((w, h, kind, seed * 1000 + 2 * i, n_src, seed * 1000 + 2 * i + 1, 32.0, True))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         return list(ex.map(lambda s: _make(*s), specs))
 
 
-def serving_requests(n: int, rate_per_s: float, seed: int = 5050, lo: int = 128, hi: int = 2048,
+def serving_requests(n: int, rate_per_s: float, seed: int = 5050, lo: int = 128, hi: int = 4096,
                      workers: int = 8):
     """(arrival_times, instances): square sides log-uniform in [lo, hi], Poisson arrivals."""
     u = _draws(seed, 4 * n).reshape(n, 4)
@@ -101,7 +101,7 @@ def serving_requests(n: int, rate_per_s: float, seed: int = 5050, lo: int = 128,
         side = max(64, min(hi, side))
         n_src = 1 + min(int(u[i, 1] * 4), 3)
         kind = "ring" if u[i, 2] < 0.5 else "smooth"
-        specs.append((side, side, kind, seed * 1000 + 2 * i, n_src, seed * 1000 + 2 * i + 1, 8.0, True))
+        specs.append((side, side, kind, seed * 1000 + 2 * i, n_src, seed * 1000 + 2 * i + 1, 32.0, True))
     gaps = -np.log(1.0 - u[:, 3]) / rate_per_s
     arrivals = np.cumsum(gaps) - gaps[0]
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:

@@ -155,3 +155,69 @@ def test_autotune_picks_fastest_converged():
                                                                best["kernel"], best["n_active"])
         t, _ = ctx.solve(cost, seeds)
         assert np.max(np.abs(t.astype(np.float64) - ref)) <= 1e-4 * ref.max()
+
+
+@pytest.fixture(scope="module")
+def contrast_4096():
+    """The 7-ring geometry scaled x16 on 4096^2, walls at speed 1e-4, one central source."""
+    speed = g.gen_ring_speed(4096, 4096, 5, 1e-4, 16.0)
+    seeds = [(2048, 2048)]
+    return speed, seeds, oracle.fmm_f64(1.0 / speed, seeds)
+
+
+@pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
+def test_c3_contrast_4096_vs_fp64(contrast_4096, kind, tol):
+    """1e-4..1 speed contrast at full size: every storage precision against the FP64 FMM."""
+    speed, seeds, ref64 = contrast_4096
+    field = {"speed_f32": lambda: speed.astype(np.float32),
+             "speed_f16": lambda: speed.astype(np.float16),
+             "speed_bf16": lambda: g.to_bf16(speed.astype(np.float32))}[kind]()
+    print(f"case c3 contrast 4096 {kind}", flush=True)
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
+                   inner_iter_cap=1) as ctx:
+        t, st = ctx.solve(field, seeds, kind=g.KINDS[kind])
+    m = oracle.error_metrics(t, ref64)
+    print(f"  {kind}: linf={m['linf']:.3e} l1={m['l1']:.3e} max={m['max_ref']:.4g} "
+          f"linf_rel={m['linf'] / m['max_ref']:.2e} waves={st['waves']}", flush=True)
+    assert st["converged"] == 1 and m["finiteness_mismatch"] == 0
+    assert m["linf"] <= tol * m["max_ref"]
+
+
+def test_c3_fp32_vs_cpu_fim(c2):
+    """The 4096^2 smooth field: the HIP path against the oracle FP32 FIM (the reference's
+    second CPU solver), and both against the FP64 FMM."""
+    speed, cost, seeds = c2
+    ref_fim, _ = oracle.fim_f32(cost, seeds, 1e-5)
+    ref64 = oracle.fmm_f64(1.0 / speed, seeds)
+    print("case c3 fp32 vs fim", flush=True)
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
+                   inner_iter_cap=1) as ctx:
+        t, _ = ctx.solve(cost, seeds)
+    d_fim = float(np.max(np.abs(t.astype(np.float64) - ref_fim)))
+    m_gpu = oracle.error_metrics(t, ref64)
+    m_fim = oracle.error_metrics(ref_fim, ref64)
+    print(f"  gpu vs fim32 linf={d_fim:.3e}; gpu vs fmm64 linf={m_gpu['linf']:.3e} "
+          f"l1={m_gpu['l1']:.3e}; fim32 vs fmm64 linf={m_fim['linf']:.3e} l1={m_fim['l1']:.3e}",
+          flush=True)
+    assert d_fim <= 1e-4 * float(ref64.max())
+    assert m_gpu["linf"] <= 1e-4 * m_gpu["max_ref"]
+
+
+def test_graph_cache_distinguishes_kernels_with_equal_tile_counts():
+    """16x16 and 32x8 tiles give the same tile count (and arguments) on a 512x256 field; a
+    captured graph of one must not be replayed for the other."""
+    speed = g.gen_smooth_speed(512, 256, 81, 8.0, 0.1, 10.0)
+    seeds = g.gen_points(512, 256, 82, 2)
+    cost = g.cost_from_speed(speed)
+    ref = oracle.fmm_f32(cost, seeds)
+    with g.Context(0, kernel=g.KERNEL_SYSTOLIC, n_active=4096, launch_mode=2) as ctx:
+        dc = ctx.to_device(cost)
+        dt = ctx.alloc(cost.shape, np.float32)
+        for tile in ((16, 16), (32, 8), (16, 16), (32, 8)):
+            print(f"case graph cache tile={tile}", flush=True)
+            ctx.set_config(tile_w=tile[0], tile_h=tile[1])
+            st = ctx.solve_device(dc.ptr, g.FIELD_COST_F32, 512, 256, seeds, dt.ptr)
+            t = dt.download()
+            assert st["converged"] == 1
+            assert st["tile_visits"] >= st["tiles_total"] // 4
+            assert np.max(np.abs(t.astype(np.float64) - ref)) <= 1e-4 * ref.max(), tile
