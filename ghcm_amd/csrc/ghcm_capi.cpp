@@ -334,8 +334,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     uint64_t max_cells = 0;
     for (uint32_t i = 0; i < n_inst; ++i) max_cells = std::max<uint64_t>(max_cells, (uint64_t)hi[i].w * hi[i].h);
     volatile HostRec* hr = ctx->rec.as<HostRec>();
-    hr->ticks = 0;
-    hr->done = 0;
+    hr->progress = 0ull;
 
     SolveArgs a{};
     a.inst = ctx->inst.as<GhcmInst>();
@@ -368,7 +367,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     auto publish_totals = [&]() -> int {
         const uint32_t tag = ++ctx->solve_seq;
         HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
-        HIP_TRY(launch_publish_totals(ctx->totals.as<unsigned long long>(), ctx->rec.dev_as<HostRec>(), tag, st));
+        HIP_TRY(launch_publish_totals(ctx->totals.as<unsigned long long>(), a.ctrl,
+                                      ctx->rec.dev_as<HostRec>(), tag, st));
         for (uint64_t spins = 0; hr->totals_tag != tag; ++spins) {
             if ((spins & 4095u) == 4095u) {
                 const hipError_t q = hipStreamQuery(st);
@@ -466,9 +466,10 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         }
         uint64_t enqueued = 0;                      // waves queued after the first boundary
         for (uint64_t spins = 0;; ++spins) {
-            if (hr->done) { converged = true; break; }
+            const unsigned long long progress = hr->progress;
+            if (progress >> 32) { converged = true; break; }
             if (enqueued >= max_waves) break;
-            const uint32_t ticks = hr->ticks;
+            const uint32_t ticks = (uint32_t)progress;
             const uint64_t executed = ticks ? ticks - 1u : 0u;
             if (enqueued < executed + lead) {
                 if (use_graph) {
@@ -506,7 +507,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
                 ++kern_launches;
             }
         }
-        converged = hr->done != 0;
+        converged = (hr->progress >> 32) != 0;
         waves_done = hr->busy_waves;
     } else {
         // ---- host priority queue (src/eikonal_hcm.cpp:343-482) ----

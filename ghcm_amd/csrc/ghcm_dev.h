@@ -88,10 +88,11 @@ struct HostStatus {
 // system-scope stores on every launch; the host polls it instead of copying the control
 // block back or waiting on events, so the wave loop has no runtime copy in it.
 struct HostRec {
-    uint32_t ticks;                 // k_wave_end launches executed
-    uint32_t done;                  // the loop found its list empty
-    uint32_t wave;
-    uint32_t busy_waves;
+    // (done << 32) | ticks: k_wave_end launches executed, and whether the loop found its
+    // list empty; one 64-bit store per launch
+    unsigned long long progress;
+    uint32_t wave;                  // refreshed every 16 ticks, when done is set and by
+    uint32_t busy_waves;            // k_publish_totals
     uint32_t totals_tag;            // set by k_publish_totals once totals[] is written
     uint32_t pad[11];
     unsigned long long totals[3];   // tile visits, in-tile passes, cells improved

@@ -41,8 +41,8 @@ hipError_t launch_wake(GhcmCtrl* ctrl, hipStream_t s);
 hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_t n_inst,
                              uint64_t max_cells, uint32_t n_select, unsigned long long* totals,
                              hipStream_t s);
-hipError_t launch_publish_totals(const unsigned long long* totals, HostRec* rec, uint32_t tag,
-                                 hipStream_t s);
+hipError_t launch_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
+                                 HostRec* rec, uint32_t tag, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
                        uint32_t tile_w, uint32_t tile_h, hipStream_t s);

@@ -906,7 +906,7 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
         if (a.rec && tid == 0) {
             const uint32_t t = a.ctrl->ticks + 1u;
             a.ctrl->ticks = t;
-            sys_store(&a.rec->ticks, t);
+            sys_store64(&a.rec->progress, (1ull << 32) | t);
         }
         return;
     }
@@ -932,11 +932,13 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     if (tid == 0) {
         GhcmCtrl* c = a.ctrl;
         const uint32_t t = c->ticks + 1u;
+        const uint32_t done = c->done;
         c->ticks = t;
-        sys_store(&a.rec->wave, c->wave);
-        sys_store(&a.rec->busy_waves, c->busy_waves);
-        sys_store(&a.rec->done, c->done);
-        sys_store(&a.rec->ticks, t);
+        if ((t & 15u) == 0u || done) {
+            sys_store(&a.rec->wave, c->wave);
+            sys_store(&a.rec->busy_waves, c->busy_waves);
+        }
+        sys_store64(&a.rec->progress, ((unsigned long long)done << 32) | t);
     }
 }
 
@@ -981,9 +983,12 @@ __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_ins
 }
 
 // Totals of the per-tile counters handed to the host record; the tag is written last.
-__global__ void k_publish_totals(const unsigned long long* totals, HostRec* rec, uint32_t tag) {
+__global__ void k_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
+                                 HostRec* rec, uint32_t tag) {
     if (threadIdx.x == 0) {
         for (int j = 0; j < 3; ++j) sys_store64(&rec->totals[j], totals[j]);
+        sys_store(&rec->wave, ctrl->wave);
+        sys_store(&rec->busy_waves, ctrl->busy_waves);
         sys_store(&rec->totals_tag, tag);
     }
 }
@@ -1398,9 +1403,9 @@ hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_publish_totals(const unsigned long long* totals, HostRec* rec, uint32_t tag,
-                                 hipStream_t s) {
-    hipLaunchKernelGGL(k_publish_totals, dim3(1), dim3(64), 0, s, totals, rec, tag);
+hipError_t launch_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
+                                 HostRec* rec, uint32_t tag, hipStream_t s) {
+    hipLaunchKernelGGL(k_publish_totals, dim3(1), dim3(64), 0, s, totals, ctrl, rec, tag);
     return hipGetLastError();
 }
 

@@ -530,7 +530,7 @@ int ghcm_server::loop() {
             std::lock_guard<std::mutex> lk(mu);
             st.peak_live = std::max<uint64_t>(st.peak_live, n_live);
         }
-        const uint32_t ticks = hr->ticks;
+        const uint32_t ticks = (uint32_t)hr->progress;
         // Keep `lead` waves queued ahead of the last executed boundary while work is live.
         if (n_live > 0 && (uint32_t)(enqueued - ticks) < lead) {
             int rc = enqueue_step();
