@@ -471,8 +471,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 // the current row was produced by lane L-1 one step earlier: it arrives through a
 // wave_shr:1 DPP move, and the upwind row neighbour is the lane's own previous result.
 // The other neighbours are read from LDS (any value there is an upper bound, possibly
-// already lowered by another wave), and results are merged with LDS atomicMin.  A round is
-// the four sweeps; rounds repeat until no cell of the tile improves by >= eps.
+// already lowered by another wave), and improvements are written back with plain LDS
+// stores.  A round is the four sweeps; rounds repeat until no cell of the tile improves by
+// >= eps or the round cap is reached (the tile then re-queues itself).
 // ---------------------------------------------------------------------------------------
 constexpr int kSweepThreads = 256;
 constexpr uint32_t kSweepHardRounds = 64;
