@@ -70,6 +70,17 @@ __device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
     return fminf(t, r);
 }
 
+// The same update without the final min against the old time, for callers that compare
+// against it themselves; |ax - ay| takes the place of max - min.
+__device__ __forceinline__ float godunov_raw(float ax, float ay, float c) {
+    const float m = fminf(ax, ay);
+    const float d = fabsf(ax - ay);                // NaN when both are +INF
+    const float s = 1.41421356237309515f * c;
+    // (s - d)(s + d) >= 0 whenever c >= d; the other branch discards it.
+    const float two = 0.5f * (d + __builtin_amdgcn_sqrtf((s - d) * (s + d)));
+    return m + ((c >= d) ? two : c);
+}
+
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
@@ -478,9 +489,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 constexpr int kSweepThreads = 256;
 constexpr uint32_t kSweepHardRounds = 64;
 
-__device__ __forceinline__ float dpp_from_lower_lane(float v) {
-    // wave_shr:1 -- lane i receives lane i-1; lane 0 receives +INF.
-    return __int_as_float(__builtin_amdgcn_update_dpp((int)kInfBits, __float_as_int(v), 0x138,
+__device__ __forceinline__ float dpp_from_lower_lane(float old, float v) {
+    // wave_shr:1 -- lane i receives lane i-1's v; lane 0 keeps `old`.
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138,
                                                       0xF, 0xF, false));
 }
 
@@ -525,72 +536,91 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const float eps = a.eps;
     const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kSweepHardRounds) ? kSweepHardRounds : a.iter_cap;
     const int dx = (w & 1) ? -1 : 1;
-    const int inc = (w & 2) ? -PW : PW;            // row step of this wave's sweep
+    const bool down = (w & 2) == 0;                // rows ascending
     const int x = (dx > 0) ? L : TW - 1 - L;
     const bool lane_on = L < TW;
-    // Cell of this lane at step s: row index yi = s - L (rows ascending for inc > 0,
-    // descending otherwise).  b0 is the top-left corner of its 3x3 neighbourhood at s = 0;
-    // all six reads of a step are immediate offsets from one corner index.
-    const int q0 = ((inc > 0) ? (1 - L) : (TH + L)) * PW + (x + 1);
-    const int b0 = q0 - PW - 1;
-    constexpr int kDummy = 0;                      // corner of padded cell (1,1): in range
-    constexpr int kSinkB = PP - (PW + 1);          // corner whose centre is the sink word
+    // Lanes past the tile width (TW < 64) shadow the last column and never store.
+    const int xc = lane_on ? x : ((dx > 0) ? TW - 1 : 0);
+    // Wrapped skew: at step s lane L handles pass position (s - L) mod TH of its column, i.e.
+    // row (s - L) mod TH of a downward sweep or TH-1 minus it of an upward one.  The upwind
+    // column neighbour of that row was produced by lane L-1 one step earlier and arrives
+    // through DPP; the upwind row neighbour is the lane's own previous result, except at the
+    // first row of a pass, whose upwind neighbour is the halo in LDS.  No lane ever idles:
+    // before and after its ordered pass a lane relaxes cells of the neighbouring passes,
+    // which is valid because every value involved is an upper bound.  TW + TH - 1 steps give
+    // every column one complete ordered pass.
+    static_assert((TH & (TH - 1)) == 0, "TH must be a power of two");
+    const int r_first = down ? 0 : TH - 1;
+    const int step_dir = down ? 1 : -1;
+    // Every round starts from the same positions: lane L reaches pass position 0 at step L,
+    // one step after lane L-1, so the ordered passes of the lanes form one sweep front.
+    const int p0 = (-L) & (TH - 1);
+    const int r_start = down ? p0 : TH - 1 - p0;
+    int r = r_start;
     uint32_t rounds = 0;
     bool unfinished = false;
-    // Step s reads only cells whose values are upper bounds whatever the interleaving, so the
-    // LDS reads of step s+1 are issued before step s is computed (software pipelining, two
-    // register sets so no copy waits on a read in flight): the dependent chain per step is
-    // then DPP + Godunov update only.  The min over each axis is symmetric, so the reads do
-    // not depend on the sweep direction; the upwind values also arrive fresh through DPP
-    // (column) and `prev` (row).  Steps past the last row are inactive for every lane.
+    // LDS reads of step s+1 are issued before step s is computed (two register sets): their
+    // cells are upper bounds whatever the interleaving, and the fresh upwind values arrive
+    // through DPP and `prev`, so the dependent chain per step is DPP + update.
     constexpr int S = TW + TH - 1;
     constexpr int SE = (S + 1) & ~1;
+    // Byte offsets into the LDS tile: corner of the 3x3 around (r+1, xc+1), so every read
+    // and the store are immediate offsets from one address.
+    char* const shb = reinterpret_cast<char*>(sh);
+    const int xc4 = 4 * xc;
     struct Nb {
         float tc, cc, tl, tr, tu, td;
         int b;
-        bool act;
+        bool first;
     };
-    auto load = [&](int st, Nb& o) {
-        o.act = lane_on && (unsigned)(st - L) < (unsigned)TH;
-        o.b = b0 + st * inc;
-        const float* r = sh + (o.act ? o.b : kDummy);
-        o.tu = r[1];
-        o.tl = r[PW];
-        o.tc = r[PW + 1];
-        o.tr = r[PW + 2];
-        o.td = r[2 * PW + 1];
-        o.cc = r[CB + PW + 1];
+    auto at = [&](int b, int word) -> float { return *reinterpret_cast<const float*>(shb + b + 4 * word); };
+    auto load = [&](Nb& o) {
+        o.b = r * (4 * PW) + xc4;
+        o.first = r == r_first;
+        o.tu = at(o.b, 1);
+        o.tl = at(o.b, PW);
+        o.tc = at(o.b, PW + 1);
+        o.tr = at(o.b, PW + 2);
+        o.td = at(o.b, 2 * PW + 1);
+        o.cc = at(o.b, CB + PW + 1);
+        r = (r + step_dir) & (TH - 1);
     };
     for (;;) {
         float prev = __builtin_inff();
-        bool changed = false;
+        float maxdec = 0.0f;
+        r = r_start;
         auto step = [&](const Nb& n) {
-            const float left = dpp_from_lower_lane(prev);
+            // Lane L-1's fresh value of the left cell (lane 0: the LDS value, its halo).
+            const float left = dpp_from_lower_lane(n.tl, prev);
             const float ax = fminf(fminf(n.tl, left), n.tr);
-            const float ay = fminf(fminf(n.tu, prev), n.td);
-            const float nv = godunov(ax, ay, n.cc, n.tc);
-            // Plain store of an improvement.  Another wave may have lowered the cell since it
-            // was read, and this store can raise it back to a value that is still an upper
-            // bound; the round only ends when no sweep improves any cell, so a lost update
-            // costs at most another round.  Other lanes store into the sink word.
-            const bool imp = n.act & (nv < n.tc);
-            sh[(imp ? n.b : kSinkB) + PW + 1] = nv;
-            const float dec = n.tc - nv;                     // >= 0; NaN when both are INF
-            changed |= imp & (dec >= eps);
-            prev = n.act ? nv : __builtin_inff();
+            const float up = n.first ? __builtin_inff() : prev;
+            const float ay = fminf(fminf(n.tu, up), n.td);
+            const float t = godunov_raw(ax, ay, n.cc);
+            // Merge into the cell with an LDS min (no return value is waited for): the four
+            // sweeps write the same tile concurrently and a min never raises a cell another
+            // sweep has just lowered.  Non-negative floats order like their bit patterns.
+            const bool lt = t < n.tc;
+            const float nv = lt ? t : n.tc;
+            if (TW == 64 || lane_on)
+                __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * (PW + 1)),
+                                       __float_as_uint(t), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            maxdec = fmaxf(maxdec, n.tc - nv);       // NaN (both +INF) is ignored by fmaxf
+            prev = nv;
         };
         Nb A, B;
-        load(0, A);
+        load(A);
         for (int st = 0; st < SE; st += 2) {
-            load(st + 1, B);
+            load(B);
             __builtin_amdgcn_sched_barrier(0);
             step(A);
-            load(st + 2, A);
+            load(A);
             __builtin_amdgcn_sched_barrier(0);
             step(B);
         }
         ++rounds;
-        const bool any = __ballot(changed) != 0ull;
+        // A strict improvement of at least eps (eps may be 0).
+        const bool any = __ballot(lane_on && maxdec > 0.0f && maxdec >= eps) != 0ull;
         if (L == 0) s_u32[w] = any ? 1u : 0u;
         __syncthreads();
         const bool more = (s_u32[0] | s_u32[1] | s_u32[2] | s_u32[3]) != 0u;
