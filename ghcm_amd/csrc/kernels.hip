@@ -71,9 +71,8 @@ __device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
 }
 
 // The same update without the final min against the old time, for callers that compare
-// against it themselves; |ax - ay| takes the place of max - min.
-__device__ __forceinline__ float godunov_raw(float ax, float ay, float c) {
-    const float m = fminf(ax, ay);
+// against it themselves; m = min(ax, ay) is passed in, |ax - ay| takes the place of max - min.
+__device__ __forceinline__ float godunov_raw(float ax, float ay, float m, float c) {
     const float d = fabsf(ax - ay);                // NaN when both are +INF
     const float s = 1.41421356237309515f * c;
     // (s - d)(s + d) >= 0 whenever c >= d; the other branch discards it.
@@ -99,6 +98,12 @@ __device__ __forceinline__ void sys_store(uint32_t* p, uint32_t v) {
 }
 __device__ __forceinline__ void sys_store64(unsigned long long* p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Per-tile counter increment without waiting for a return value (one writer per tile per
+// wave, so the atomic only saves the load of a read-modify-write).
+__device__ __forceinline__ void stat_add(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Append tile g to list `which` unless it is already there (qwave), in shard g % kShards.
@@ -411,16 +416,16 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             st.requeue = unfinished ? 1u : 0u;
             a.status[job_slot] = st;
             a.tmin[g] = tmin;
-            a.tstat[g] += 1u;
-            a.tstat[(size_t)a.tiles + g] += passes;
-            a.tstat[2 * (size_t)a.tiles + g] += improved;
+            stat_add(&a.tstat[g], 1u);
+            stat_add(&a.tstat[(size_t)a.tiles + g], passes);
+            stat_add(&a.tstat[2 * (size_t)a.tiles + g], improved);
         }
     } else {
         if (lane == 0) {
             a.tmin[g] = tmin;
-            a.tstat[g] += 1u;
-            a.tstat[(size_t)a.tiles + g] += passes;
-            a.tstat[2 * (size_t)a.tiles + g] += improved;
+            stat_add(&a.tstat[g], 1u);
+            stat_add(&a.tstat[(size_t)a.tiles + g], passes);
+            stat_add(&a.tstat[2 * (size_t)a.tiles + g], improved);
             if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tmin);
         }
         if (lane < 4 && ((side_flags >> lane) & 1u)) {
@@ -502,7 +507,7 @@ constexpr int sweep_lds_words() { return 2 * (TW + 2) * (TH + 2) + 1; }
 template <int TW, int TH, int KIND, bool HOST>
 __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
                            uint32_t p, uint32_t job_slot, float* sh,
-                           uint32_t* s_u32, float* s_f32) {
+                           uint32_t* s_u32, float* s_f32, const GhcmInst& in_single) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
     constexpr int PP = PW * PH;
@@ -515,8 +520,8 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const int tid = threadIdx.x;
     const int w = tid >> 6, L = tid & 63;
 
-    const uint32_t ii = (a.n_inst == 1) ? 0u : a.tinst[g];
-    const GhcmInst in = a.inst[ii];
+    // One instance: its record was loaded before the job loop (one dependent load fewer).
+    const GhcmInst in = (a.n_inst == 1) ? in_single : a.inst[a.tinst[g]];
     const uint32_t lt = g - in.tile_base;
     const int tx = (int)(lt % in.tiles_x);
     const int ty = (int)(lt / in.tiles_x);
@@ -589,18 +594,22 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         float prev = __builtin_inff();
         float maxdec = 0.0f;
         r = r_start;
+        // Times are non-negative or +INF, so their bit patterns order like the values: the
+        // axis minima and the merge use integer minimum (no NaN canonicalisation needed).
+        auto umin = [](float u, float v) {
+            return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
+        };
         auto step = [&](const Nb& n) {
             // Lane L-1's fresh value of the left cell (lane 0: the LDS value, its halo).
             const float left = dpp_from_lower_lane(n.tl, prev);
-            const float ax = fminf(fminf(n.tl, left), n.tr);
+            const float ax = umin(umin(n.tl, left), n.tr);
             const float up = n.first ? __builtin_inff() : prev;
-            const float ay = fminf(fminf(n.tu, up), n.td);
-            const float t = godunov_raw(ax, ay, n.cc);
+            const float ay = umin(umin(n.tu, up), n.td);
+            const float t = godunov_raw(ax, ay, umin(ax, ay), n.cc);
             // Merge into the cell with an LDS min (no return value is waited for): the four
             // sweeps write the same tile concurrently and a min never raises a cell another
             // sweep has just lowered.  Non-negative floats order like their bit patterns.
-            const bool lt = t < n.tc;
-            const float nv = lt ? t : n.tc;
+            const float nv = umin(t, n.tc);
             if (TW == 64 || lane_on)
                 __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * (PW + 1)),
                                        __float_as_uint(t), __ATOMIC_RELAXED,
@@ -610,6 +619,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         };
         Nb A, B;
         load(A);
+#pragma unroll 2
         for (int st = 0; st < SE; st += 2) {
             load(B);
             __builtin_amdgcn_sched_barrier(0);
@@ -666,6 +676,18 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             s_f32[side] = kc;
         }
     }
+    if constexpr (!HOST) {
+        // Activate the dirty sides' neighbours now, so the atomics' round trips overlap the
+        // commit below; the neighbours run in a later wave, after this kernel's stores.
+        __syncthreads();
+        if (tid < 4 && s_u32[4 + tid]) {
+            const int sd = tid;
+            const int nx = tx + (sd == 0 ? -1 : sd == 2 ? 1 : 0);
+            const int ny = ty + (sd == 1 ? -1 : sd == 3 ? 1 : 0);
+            const uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
+            activate(a, nb, wave, cur, p, 1u << (sd ^ 2), s_f32[sd]);
+        }
+    }
 
     // Commit improved cells (plain stores: one writer per tile per wave), tile min.
     float tmin = __builtin_inff();
@@ -696,9 +718,9 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         const float tm = fminf(fminf(s_f32[4], s_f32[5]), fminf(s_f32[6], s_f32[7]));
         const uint32_t imp = s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11];
         a.tmin[g] = tm;
-        a.tstat[g] += 1u;
-        a.tstat[(size_t)a.tiles + g] += rounds;
-        a.tstat[2 * (size_t)a.tiles + g] += imp;
+        stat_add(&a.tstat[g], 1u);
+        stat_add(&a.tstat[(size_t)a.tiles + g], rounds);
+        stat_add(&a.tstat[2 * (size_t)a.tiles + g], imp);
         if constexpr (HOST) {
             HostStatus st;
             st.solves = rounds;
@@ -715,15 +737,6 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tm);
         }
     }
-    if constexpr (!HOST) {
-        if (tid < 4 && s_u32[4 + tid]) {
-            const int sd = tid;
-            const int nx = tx + (sd == 0 ? -1 : sd == 2 ? 1 : 0);
-            const int ny = ty + (sd == 1 ? -1 : sd == 3 ? 1 : 0);
-            const uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
-            activate(a, nb, wave, cur, p, 1u << (sd ^ 2), s_f32[sd]);
-        }
-    }
 }
 
 template <int TW, int TH, int KIND, bool HOST>
@@ -732,26 +745,30 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
     __shared__ uint32_t s_u32[16];
     __shared__ float s_f32[8];
     const int tid = threadIdx.x;
+    // Single-instance solves: the instance record is read once, next to the control block.
+    GhcmInst in0{};
+    if (a.n_inst == 1) in0 = a.inst[0];
     if constexpr (HOST) {
         const uint32_t j = blockIdx.x;
         if (j >= a.n_host_jobs) return;
-        sweep_tile<TW, TH, KIND, true>(a, a.host_jobs[j].id, 0u, 0u, 0u, j, sh, s_u32, s_f32);
+        sweep_tile<TW, TH, KIND, true>(a, a.host_jobs[j].id, 0u, 0u, 0u, j, sh, s_u32, s_f32, in0);
     } else {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
-        if (ctrl->done) return;
+        // All control words are read together; the done check follows.
+        const uint32_t done = ctrl->done;
         const uint32_t nj = ctrl->n_jobs;
-        if (blockIdx.x >= nj) return;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
+        if (done || blockIdx.x >= nj) return;
         const uint32_t p = wave & 1u;
         for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
         __syncthreads();
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
     }
@@ -1121,6 +1138,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
     uint32_t* hist = reinterpret_cast<uint32_t*>(sh);
     uint32_t* part = reinterpret_cast<uint32_t*>(sh) + 4096;
     GhcmCtrl* ctrl = a.ctrl;
+    GhcmInst in0{};
+    if (a.n_inst == 1) in0 = a.inst[0];
     const int tid = threadIdx.x;
     const uint32_t G = gridDim.x;
     const uint32_t n_groups = (G + kBarGroup - 1) / kBarGroup;
@@ -1143,7 +1162,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
         for (uint32_t j = blockIdx.x; j < nj; j += G) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
         // ---- arrive ----
@@ -1359,6 +1378,7 @@ static SolveKernel pick_mode(int kind, int mode, bool host) {
 
 SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host) {
     if (mode == 2) {
+        if (tw == 64 && th == 128) return pick_sweep<64, 128>(kind, host);
         if (tw == 64 && th == 64) return pick_sweep<64, 64>(kind, host);
         if (tw == 64 && th == 32) return pick_sweep<64, 32>(kind, host);
         if (tw == 64 && th == 16) return pick_sweep<64, 16>(kind, host);
