@@ -106,6 +106,20 @@ __device__ __forceinline__ void stat_add(uint32_t* p, uint32_t v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Progress record for the host (done flag and the count of wave boundaries executed),
+// written by the first work-group of each tile-solve launch: the store to host memory
+// then overlaps a tile visit instead of ending k_wave_end.  The wave counters follow every
+// 16 boundaries and once the loop is done.
+__device__ __forceinline__ void publish_progress(const SolveArgs& a, const GhcmCtrl* ctrl) {
+    if (!a.rec || blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t t = ctrl->ticks, done = ctrl->done;
+    if ((t & 15u) == 0u || done) {
+        sys_store(&a.rec->wave, ctrl->wave);
+        sys_store(&a.rec->busy_waves, ctrl->busy_waves);
+    }
+    sys_store64(&a.rec->progress, ((unsigned long long)done << 32) | t);
+}
+
 // Append tile g to list `which` unless it is already there (qwave), in shard g % kShards.
 __device__ __forceinline__ void list_append(const SolveArgs& a, uint32_t which, uint32_t g) {
     const uint32_t sh = g % kShards;
@@ -457,6 +471,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     } else {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
+        publish_progress(a, ctrl);
         if (ctrl->done) return;
         const uint32_t nj = ctrl->n_jobs;
         if (blockIdx.x >= nj) return;
@@ -755,6 +770,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
     } else {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
+        publish_progress(a, ctrl);
         // All control words are read together; the done check follows.
         const uint32_t done = ctrl->done;
         const uint32_t nj = ctrl->n_jobs;
@@ -951,15 +967,12 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     const int tid = threadIdx.x;
     if (a.ctrl->done) {
         // Idle launch: only the tick advances, so the host still sees progress.
-        if (a.rec && tid == 0) {
-            const uint32_t t = a.ctrl->ticks + 1u;
-            a.ctrl->ticks = t;
-            sys_store64(&a.rec->progress, (1ull << 32) | t);
-        }
+        if (tid == 0) a.ctrl->ticks += 1u;
         return;
     }
     const uint32_t cur0 = a.ctrl->cur;
     wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
+    if (tid == 0) a.ctrl->ticks += 1u;
     if (!a.rec) return;
     __syncthreads();
     if (a.n_slots) {
@@ -976,17 +989,6 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
         __syncthreads();
         for (uint32_t i = tid; i < ns; i += 1024)
             sys_store64(&a.rec->slot[i], ((unsigned long long)a.sgen[i] << 32) | hist[i]);
-    }
-    if (tid == 0) {
-        GhcmCtrl* c = a.ctrl;
-        const uint32_t t = c->ticks + 1u;
-        const uint32_t done = c->done;
-        c->ticks = t;
-        if ((t & 15u) == 0u || done) {
-            sys_store(&a.rec->wave, c->wave);
-            sys_store(&a.rec->busy_waves, c->busy_waves);
-        }
-        sys_store64(&a.rec->progress, ((unsigned long long)done << 32) | t);
     }
 }
 
