@@ -115,7 +115,12 @@ class ServeStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
-# Every symbol declared by include/ghcm.h and include/eikonal_c.h.
+class Image(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("comp", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("data", ctypes.POINTER(ctypes.c_float))]
+
+
+# Every symbol declared by include/ghcm.h, include/eikonal_c.h and include/ghcm_io.h.
 EXPORTED_SYMBOLS = (
     "ghcm_abi_version", "ghcm_status_string", "ghcm_last_error", "ghcm_config_default",
     "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
@@ -127,6 +132,10 @@ EXPORTED_SYMBOLS = (
     "ghcm_serve_get_stats", "ghcm_serve_stop",
     "ghcm_gen_ring_speed", "ghcm_gen_smooth_speed", "ghcm_gen_points",
     "ghcm_cpu_fmm", "ghcm_cpu_fim", "ghcm_hcm_solve_cxx",
+    "ghcm_image_free", "ghcm_image_load_f32", "ghcm_png_decode8", "ghcm_png_write8",
+    "ghcm_field_load_mono", "ghcm_field_load_rgb_luminosity", "ghcm_rgb_luminosity_device",
+    "ghcm_write_time_field_png", "ghcm_write_field_hdr", "ghcm_blob_write", "ghcm_blob_read",
+    "ghcm_test_run", "ghcm_field_cost_sin_checkerboard", "ghcm_field_cost_random",
 )
 
 _lib = None
@@ -187,6 +196,24 @@ def lib() -> ctypes.CDLL:
     L.ghcm_cpu_fim.argtypes = [ctypes.c_float, vp, u32, u32, vp, u32, vp, P(ctypes.c_double)]
     L.ghcm_hcm_solve_cxx.argtypes = [i32, u32, ctypes.c_float, ctypes.c_float, i32, vp,
                                      u32, u32, vp, u32, vp, P(Stats)]
+    PI = P(Image)
+    L.ghcm_image_free.argtypes = [PI]
+    L.ghcm_image_free.restype = None
+    L.ghcm_image_load_f32.argtypes = [ctypes.c_char_p, i32, PI]
+    L.ghcm_png_decode8.argtypes = [vp, ctypes.c_size_t, i32, P(u32), P(u32), P(u32), vp, ctypes.c_size_t]
+    L.ghcm_png_write8.argtypes = [ctypes.c_char_p, vp, u32, u32, u32]
+    L.ghcm_field_load_mono.argtypes = [ctypes.c_char_p, PI]
+    L.ghcm_field_load_rgb_luminosity.argtypes = [vp, ctypes.c_char_p, PI]
+    L.ghcm_rgb_luminosity_device.argtypes = [vp, vp, u32, u32, vp]
+    L.ghcm_write_time_field_png.argtypes = [ctypes.c_char_p, vp, u32, u32, ctypes.c_float,
+                                            ctypes.c_float, i32]
+    L.ghcm_write_field_hdr.argtypes = [ctypes.c_char_p, vp, u32, u32]
+    L.ghcm_blob_write.argtypes = [ctypes.c_char_p, vp, u32, u32]
+    L.ghcm_blob_read.argtypes = [ctypes.c_char_p, PI]
+    L.ghcm_test_run.argtypes = [ctypes.c_char_p, vp, ctypes.c_char_p, ctypes.c_char_p, vp, u32, u32,
+                                vp, u32, ctypes.c_char_p, u32, ctypes.c_float]
+    L.ghcm_field_cost_sin_checkerboard.argtypes = [u32, u32, ctypes.c_float, ctypes.c_float, vp]
+    L.ghcm_field_cost_random.argtypes = [u32, u32, ctypes.c_uint64, vp]
     for name in EXPORTED_SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or i32
     _lib = L
@@ -570,3 +597,112 @@ def cost_from_speed(speed: np.ndarray) -> np.ndarray:
     """cost = 1/speed in FP32 (src/test_runner.cpp:32-36); speed 0 -> +INF."""
     with np.errstate(divide="ignore"):
         return (np.float32(1.0) / np.asarray(speed, dtype=np.float32)).astype(np.float32)
+
+
+# ---- field loaders, writers and the test runner (include/ghcm_io.h) -------------------
+
+def _take_image(img: Image) -> np.ndarray:
+    n = img.w * img.h * img.comp
+    arr = np.ctypeslib.as_array(img.data, shape=(n,)).copy() if n else np.zeros(0, np.float32)
+    lib().ghcm_image_free(ctypes.byref(img))
+    return arr
+
+
+def _path(p) -> bytes:
+    return os.fsencode(p)
+
+
+def png_decode8(data: bytes, req_comp: int = 0) -> np.ndarray:
+    """PNG bytes -> (H, W, C) uint8, converted as stb_image does for `req_comp` channels."""
+    L = lib()
+    w, h, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    _check(L.ghcm_png_decode8(buf, len(data), req_comp, ctypes.byref(w), ctypes.byref(h),
+                              ctypes.byref(c), None, 0))
+    out = np.empty((h.value, w.value, c.value), dtype=np.uint8)
+    _check(L.ghcm_png_decode8(buf, len(data), req_comp, ctypes.byref(w), ctypes.byref(h),
+                              ctypes.byref(c), out.ctypes.data, out.nbytes))
+    return out
+
+
+def png_write8(path, pixels: np.ndarray) -> None:
+    p = np.ascontiguousarray(pixels, dtype=np.uint8)
+    if p.ndim == 2:
+        p = p[:, :, None]
+    _check(lib().ghcm_png_write8(_path(path), p.ctypes.data, p.shape[1], p.shape[0], p.shape[2]))
+
+
+def image_load_f32(path, req_comp: int) -> np.ndarray:
+    """stbi_loadf(path, req_comp) equivalent: (H, W, C) float32."""
+    img = Image()
+    _check(lib().ghcm_image_load_f32(_path(path), req_comp, ctypes.byref(img)))
+    w, h, c = img.w, img.h, img.comp
+    return _take_image(img).reshape(h, w, c)
+
+
+def field_load_mono(path) -> np.ndarray:
+    """field2d_img_monochannel (src/field_loaders.cpp:53-59): (H, W) float32."""
+    img = Image()
+    _check(lib().ghcm_field_load_mono(_path(path), ctypes.byref(img)))
+    w, h = img.w, img.h
+    return _take_image(img).reshape(h, w)
+
+
+def field_load_rgb_luminosity(ctx: "Context", path) -> np.ndarray:
+    """field2d_img_rgb_luminosity (src/field_loaders.cpp:62-102) on ctx's GPU."""
+    img = Image()
+    _check(lib().ghcm_field_load_rgb_luminosity(ctx._h, _path(path), ctypes.byref(img)))
+    w, h = img.w, img.h
+    return _take_image(img).reshape(h, w)
+
+
+def write_time_field_png(path, field: np.ndarray, f_min=None, f_max=None) -> None:
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    auto = f_min is None or f_max is None
+    _check(lib().ghcm_write_time_field_png(_path(path), f.ctypes.data, f.shape[1], f.shape[0],
+                                           0.0 if auto else f_min, 0.0 if auto else f_max,
+                                           1 if auto else 0))
+
+
+def write_field_hdr(path, field: np.ndarray) -> None:
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    _check(lib().ghcm_write_field_hdr(_path(path), f.ctypes.data, f.shape[1], f.shape[0]))
+
+
+def blob_write(path, field: np.ndarray) -> None:
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    _check(lib().ghcm_blob_write(_path(path), f.ctypes.data, f.shape[1], f.shape[0]))
+
+
+def blob_read(path) -> np.ndarray:
+    img = Image()
+    _check(lib().ghcm_blob_read(_path(path), ctypes.byref(img)))
+    w, h = img.w, img.h
+    return _take_image(img).reshape(h, w)
+
+
+def test_run(solver: str, cost: np.ndarray, seeds, out_dir, solver_name=None, job_name="job",
+             ctx: "Context" = None, trials: int = 5, fim_minimum_change: float = 1e-3) -> None:
+    """test_run (src/test_runner.cpp:163-217) for one job with solver "fmm", "fim" or "hcm"."""
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    s = _seeds_array(seeds)
+    _check(lib().ghcm_test_run(solver.encode(), ctx._h if ctx else None,
+                               (solver_name or solver).encode(), job_name.encode(), c.ctypes.data,
+                               c.shape[1], c.shape[0], s.ctypes.data, s.shape[0], _path(out_dir),
+                               trials, fim_minimum_change))
+
+
+test_run.__test__ = False   # not a pytest test
+
+
+def field_cost_sin_checkerboard(w: int, h: int, speed_amplitude: float, checkerboard_res: float):
+    out = np.empty((h, w), dtype=np.float32)
+    _check(lib().ghcm_field_cost_sin_checkerboard(w, h, speed_amplitude, checkerboard_res,
+                                                  out.ctypes.data))
+    return out
+
+
+def field_cost_random(w: int, h: int, seed: int):
+    out = np.empty((h, w), dtype=np.float32)
+    _check(lib().ghcm_field_cost_random(w, h, seed, out.ctypes.data))
+    return out

@@ -22,9 +22,11 @@ ghcm_config HcmConfig::to_c() const {
     c.n_active = n_active;
     c.min_change = fim_minimum_change;
     c.cache_mode = (int)cache_mode;
-    c.kernel = kernel == Kernel::FIM ? GHCM_KERNEL_FIM : GHCM_KERNEL_SYSTOLIC;
+    c.kernel = kernel == Kernel::FIM ? GHCM_KERNEL_FIM
+             : kernel == Kernel::Sweep ? GHCM_KERNEL_SWEEP : GHCM_KERNEL_SYSTOLIC;
     c.scheduler = scheduler == Scheduler::HostPQ ? GHCM_SCHED_HOST_PQ : GHCM_SCHED_DEVICE_TOPN;
     c.inner_iter_cap = inner_iter_cap;
+    c.launch_mode = launch_mode;
     return c;
 }
 
@@ -44,7 +46,8 @@ std::ostream& operator<<(std::ostream& os, HcmConfig const& c) {
        << "\nCompute Unit Overload : " << c.compute_unit_overload
        << "\nCache Mode            : " << mode
        << "\nTile                  : " << cc.tile_w << "x" << cc.tile_h
-       << "\nKernel                : " << (c.kernel == HcmConfig::Kernel::FIM ? "FIM" : "Systolic")
+       << "\nKernel                : " << (c.kernel == HcmConfig::Kernel::FIM ? "FIM"
+                                           : c.kernel == HcmConfig::Kernel::Sweep ? "Sweep" : "Systolic")
        << "\nScheduler             : " << (c.scheduler == HcmConfig::Scheduler::HostPQ ? "host PQ" : "device top-n")
        << "\n";
     return os;
@@ -154,7 +157,8 @@ extern "C" int ghcm_hcm_solve_cxx(int ordinal, uint32_t block_size, float cu_ove
     c.block_size = (uint8_t)block_size;
     c.compute_unit_overload = cu_overload;
     c.fim_minimum_change = eps;
-    c.kernel = kernel == 1 ? ghcm::HcmConfig::Kernel::FIM : ghcm::HcmConfig::Kernel::Systolic;
+    c.kernel = kernel == 1 ? ghcm::HcmConfig::Kernel::FIM
+             : kernel == 2 ? ghcm::HcmConfig::Kernel::Sweep : ghcm::HcmConfig::Kernel::Systolic;
     auto setup = ghcm::eikonal_hcm_hip_setup(ghcm::HcmDevice{ordinal}, c);
     if (!setup) return GHCM_ERR_INVALID_ARG;
     ghcm::EikonalResult r = ghcm::eikonal_hcm_hip_ex(*setup, f, s);

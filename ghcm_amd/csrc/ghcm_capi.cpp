@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "ghcm_dev.h"
+#include "ghcm_io.h"
 #include "kernels.h"
 
 using namespace ghcm;
@@ -724,6 +725,16 @@ extern "C" int ghcm_dev_copy(ghcm_ctx* ctx, void* dst, const void* src, size_t b
     HIP_TRY(hipSetDevice(ctx->ordinal));
     HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_rgb_luminosity_device(ghcm_ctx* ctx, const float* d_rgb, uint32_t w, uint32_t h,
+                                          float* d_out) {
+    if (!ctx || !d_rgb || !d_out) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    if (!w || !h) return fail(GHCM_ERR_INVALID_ARG, "zero-sized image");
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    HIP_TRY(launch_rgb_luminosity(d_rgb, d_out, (size_t)w * h, ctx->stream));
+    HIP_TRY(ghcm_capi::spin_sync(ctx->stream, ctx->ev_sync));
     return GHCM_OK;
 }
 

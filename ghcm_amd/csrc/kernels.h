@@ -44,6 +44,8 @@ hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_
 hipError_t launch_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
                                  HostRec* rec, uint32_t tag, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// n pixels of interleaved RGB floats -> n luminosity floats.
+hipError_t launch_rgb_luminosity(const float* rgb, float* out, size_t n, hipStream_t s);
 hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
                        uint32_t tile_w, uint32_t tile_h, hipStream_t s);
 

@@ -1263,6 +1263,36 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
     }
 }
 
+// ---- image processing: RGB -> luminosity (data/img_proc_rgb_luminosity.cl:5-19) ----
+// out = dot(rgb, (0.299, 0.587, 0.114)) per pixel, evaluated as
+// fma(b, 0.114, fma(g, 0.587, r * 0.299)).  HBM-bound: 12 bytes in, 4 out per pixel.
+// Each thread takes four pixels as three 16-byte loads and one 16-byte store when the
+// buffers are 16-byte aligned; the tail (and unaligned buffers) go pixel by pixel.
+__device__ __forceinline__ float luminosity(float r, float g, float b) {
+    return fmaf(b, 0.114f, fmaf(g, 0.587f, r * 0.299f));
+}
+
+__global__ void k_rgb_luminosity(const float* __restrict__ rgb, float* __restrict__ out, size_t n) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const bool vec = ((reinterpret_cast<uintptr_t>(rgb) | reinterpret_cast<uintptr_t>(out)) & 15u) == 0;
+    size_t done = 0;
+    if (vec) {
+        const size_t n4 = n / 4;
+        const float4* in4 = reinterpret_cast<const float4*>(rgb);
+        float4* out4 = reinterpret_cast<float4*>(out);
+        for (size_t i = tid; i < n4; i += stride) {
+            const float4 a = in4[3 * i], b = in4[3 * i + 1], c = in4[3 * i + 2];
+            // a = r0 g0 b0 r1, b = g1 b1 r2 g2, c = b2 r3 g3 b3
+            out4[i] = make_float4(luminosity(a.x, a.y, a.z), luminosity(a.w, b.x, b.y),
+                                  luminosity(b.z, b.w, c.x), luminosity(c.y, c.z, c.w));
+        }
+        done = n4 * 4;
+    }
+    for (size_t i = done + tid; i < n; i += stride)
+        out[i] = luminosity(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+}
+
 // ---- launcher table ----
 // ev0/ev1 (optional) are stamped by the dispatch itself (hipExtLaunchKernelGGL), so
 // timing a launch adds no marker packets to the stream.
@@ -1459,6 +1489,15 @@ hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_
 hipError_t launch_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
                                  HostRec* rec, uint32_t tag, hipStream_t s) {
     hipLaunchKernelGGL(k_publish_totals, dim3(1), dim3(64), 0, s, totals, ctrl, rec, tag);
+    return hipGetLastError();
+}
+
+hipError_t launch_rgb_luminosity(const float* rgb, float* out, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n / 4 + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_rgb_luminosity, dim3((unsigned)blocks), dim3(256), 0, s, rgb, out, n);
     return hipGetLastError();
 }
 

@@ -31,7 +31,7 @@
 namespace ghcm {
 
 struct HcmConfig {
-    enum class Kernel { Systolic, FIM };
+    enum class Kernel { Systolic, FIM, Sweep };   // Sweep: MI355X fast-sweeping tile solver
     enum class CacheMode { None, Time, Cost, Both };
     enum class Scheduler { DeviceTopN, HostPQ };
 
@@ -50,6 +50,7 @@ struct HcmConfig {
     uint32_t  n_active = 0;             // overrides compute_unit_overload
     uint32_t  inner_iter_cap = 0;       // in-tile passes per visit (0 = until converged)
     Scheduler scheduler = Scheduler::DeviceTopN;
+    uint32_t  launch_mode = 0;          // 0 per-wave launches, 1 persistent, 2 HIP graph
 
     ghcm_config to_c() const;
 };

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared_functions():
     names = set()
-    for h in ("ghcm.h", "eikonal_c.h"):
+    for h in ("ghcm.h", "eikonal_c.h", "ghcm_io.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(ghcm_[a-z0-9_]+)\s*\(", text):
@@ -64,10 +64,12 @@ def test_struct_layouts_match_c_compiler(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     mirrors = {"ghcm_config": g.Config, "ghcm_stats": g.Stats, "ghcm_instance": g.Instance,
+               "ghcm_image": g.Image,
                "ghcm_tune_candidate": g.TuneCandidate, "ghcm_device_info": g.DeviceInfo,
                "ghcm_serve_config": g.ServeConfig, "ghcm_serve_result": g.ServeResult,
                "ghcm_serve_stats": g.ServeStats}
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghcm.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ghcm.h"', '#include "ghcm_io.h"',
+             "int main(void) {"]
     for cname, cls in mirrors.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for fname, _ in cls._fields_:
