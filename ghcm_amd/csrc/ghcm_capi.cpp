@@ -388,6 +388,9 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     uint64_t waves_done = 0, visits = 0, inner = 0, improved = 0, kern_launches = 0;
     double kern_us = 0.0;
 
+    // launch_mode 3: tile solve and wave boundary in one launch (sweep kernel).
+    const SolveKernel fk = get_fused_kernel((int)tw, (int)th, kind, cfg.kernel);
+    const bool fused = !host_sched && cfg.launch_mode == 3 && fk.launch != nullptr;
     const PersistentKernel pk = get_persistent_kernel((int)tw, (int)th, kind, cfg.kernel);
     const bool persistent = !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
     if (persistent) {
@@ -432,6 +435,10 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         const bool prof = cfg.profile_kernels != 0;
         const uint32_t lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);
         const bool use_graph = cfg.launch_mode == 2 && !prof;
+        // The fused kernel's grid: one resident round of its (larger-LDS) work-groups.
+        const uint32_t fgrid = fused ? std::max(1u, std::min(n_sel, (uint32_t)std::max(1, fk.blocks_per_cu()) *
+                                                                      (uint32_t)std::max(1, ctx->cus)))
+                                     : 0u;
         const uint32_t step = use_graph ? std::max<uint32_t>(1, lead / 2) : 1;
         if (use_graph) {
             // key: the kernel (tile shape, field kind and style are one launcher each), its
@@ -482,8 +489,15 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
                         HIP_TRY(hipEventCreate(&ev));
                         ctx->ev_prof.push_back(ev);
                     }
-                    HIP_TRY(launch(a, grid, st, ctx->ev_prof[2 * enqueued], ctx->ev_prof[2 * enqueued + 1]));
-                    HIP_TRY(launch_wave_end(a, 0u, st));
+                    hipEvent_t e0 = ctx->ev_prof[2 * enqueued], e1 = ctx->ev_prof[2 * enqueued + 1];
+                    if (fused) {
+                        HIP_TRY(fk.launch(a, fgrid, st, e0, e1));
+                    } else {
+                        HIP_TRY(launch(a, grid, st, e0, e1));
+                        HIP_TRY(launch_wave_end(a, 0u, st));
+                    }
+                } else if (fused) {
+                    HIP_TRY(fk.launch(a, fgrid, st, nullptr, nullptr));
                 } else {
                     HIP_TRY(launch(a, grid, st, nullptr, nullptr));
                     HIP_TRY(launch_wave_end(a, 0u, st));

@@ -19,6 +19,8 @@ struct SolveKernel {
 
 // mode: 0 systolic, 1 FIM, 2 sweep.  host: job list given by the host scheduler.
 SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host);
+// Tile solve with the wave boundary folded in (one launch per wave); sweep kernel only.
+SolveKernel get_fused_kernel(int tile_w, int tile_h, int kind, int mode);
 
 // Persistent device wave loop (cooperative launch); null when not built for the combination.
 struct PersistentKernel {

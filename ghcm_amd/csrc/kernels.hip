@@ -957,6 +957,24 @@ __device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint
 
 constexpr int kSelectStage = 4096;     // list entries staged in LDS by k_wave_end
 
+// Serving: queued tiles per instance slot in the new current list `cur` (prefix in
+// s_prefix), tagged with the slot's admission generation; a count of 0 means nothing is
+// left to solve.  `cnt` is an LDS array of at least n_slots words.
+template <int NT>
+__device__ void publish_slots(const SolveArgs& a, const uint32_t* s_prefix, uint32_t cur,
+                              uint32_t* cnt) {
+    const int tid = threadIdx.x;
+    const uint32_t ns = a.n_slots;
+    for (uint32_t i = tid; i < ns; i += NT) cnt[i] = 0u;
+    __syncthreads();
+    const uint32_t c = s_prefix[kShards];
+    for (uint32_t i = tid; i < c; i += NT)
+        atomicAdd(&cnt[a.tinst[list_entry(a, s_prefix, cur, i)]], 1u);
+    __syncthreads();
+    for (uint32_t i = tid; i < ns; i += NT)
+        sys_store64(&a.rec->slot[i], ((unsigned long long)a.sgen[i] << 32) | cnt[i]);
+}
+
 __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) {
     __shared__ uint32_t hist[4096];
     __shared__ uint32_t part[128];
@@ -973,23 +991,9 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     const uint32_t cur0 = a.ctrl->cur;
     wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
     if (tid == 0) a.ctrl->ticks += 1u;
-    if (!a.rec) return;
+    if (!a.rec || !a.n_slots) return;
     __syncthreads();
-    if (a.n_slots) {
-        // Serving: queued tiles per instance slot in the new current list, tagged with the
-        // slot's admission generation (a count of 0 means nothing is left to solve).
-        // hist doubles as the counter array.
-        const uint32_t ns = a.n_slots;
-        for (uint32_t i = tid; i < ns; i += 1024) hist[i] = 0u;
-        __syncthreads();
-        const uint32_t c = s_prefix[kShards];
-        const uint32_t cur = first ? cur0 : (cur0 ^ 1u);
-        for (uint32_t i = tid; i < c; i += 1024)
-            atomicAdd(&hist[a.tinst[list_entry(a, s_prefix, cur, i)]], 1u);
-        __syncthreads();
-        for (uint32_t i = tid; i < ns; i += 1024)
-            sys_store64(&a.rec->slot[i], ((unsigned long long)a.sgen[i] << 32) | hist[i]);
-    }
+    publish_slots<1024>(a, s_prefix, first ? cur0 : (cur0 ^ 1u), hist);
 }
 
 // Per-solve initialisation in one launch (grid.y over instances): the control block and
@@ -1061,6 +1065,75 @@ __global__ void k_copy(const void* src, void* dst, size_t bytes) {
     const unsigned char* s1 = static_cast<const unsigned char*>(src);
     unsigned char* d1 = static_cast<unsigned char*>(dst);
     for (size_t i = done + tid; i < bytes; i += stride) d1[i] = s1[i];
+}
+
+// Tile solve and wave boundary in one launch (launch_mode 3): every work-group solves its
+// jobs as k_solve_sweep does, makes its writes visible device-wide (agent-scope fence)
+// and arrives on a counter; the last to arrive runs the wave boundary of k_wave_end --
+// list swap, top-n selection, tick -- with its 256 threads and the tile's LDS.  One
+// launch per wave instead of two.  The list entries the selection reads are appended by
+// wave 0 of each work-group (threads 0-3, and thread 0 for a capped tile), so thread 0's
+// fence orders them before the arrival.
+constexpr int kFusedStage = 2048;
+template <int TW, int TH>
+constexpr int fused_lds_words() {
+    return sweep_lds_words<TW, TH>() > 4096 + 128 + 2 * kFusedStage ? sweep_lds_words<TW, TH>()
+                                                                    : 4096 + 128 + 2 * kFusedStage;
+}
+
+template <int TW, int TH, int KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a) {
+    __shared__ float sh[fused_lds_words<TW, TH>()];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    __shared__ uint32_t s_prefix[kShards + 1];
+    __shared__ uint32_t s_misc[4];
+    const int tid = threadIdx.x;
+    GhcmInst in0{};
+    if (a.n_inst == 1) in0 = a.inst[0];
+    GhcmCtrl* ctrl = a.ctrl;
+    publish_progress(a, ctrl);
+    const uint32_t done = ctrl->done;
+    const uint32_t nj = ctrl->n_jobs;
+    const uint32_t wave = ctrl->wave;
+    const uint32_t cur = ctrl->cur;
+    const uint32_t from_list = ctrl->jobs_from_list;
+    if (!done && blockIdx.x < nj) {
+        const uint32_t p = wave & 1u;
+        for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
+        __syncthreads();
+        for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
+            const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+            if (tid == 0) claim_tile(a, g, p);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // Release only (L2 write-back, no invalidate): the other work-groups on this XCD
+        // keep their cached lines; only the last arrival acquires.
+        const uint32_t prev = __hip_atomic_fetch_add(&ctrl->arrive_top, 1u, __ATOMIC_RELEASE,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        s_misc[3] = (prev == gridDim.x - 1u) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_misc[3]) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tid == 0) ctrl->arrive_top = 0u;
+    if (done) {
+        if (tid == 0) ctrl->ticks += 1u;
+        return;
+    }
+    uint32_t* hist = reinterpret_cast<uint32_t*>(sh);
+    uint32_t* part = hist + 4096;
+    uint32_t* sg = part + 128;
+    uint32_t* sk = sg + kFusedStage;
+    wave_select<kSweepThreads, kFusedStage>(a, false, hist, part, s_prefix, s_misc, sg, sk);
+    if (tid == 0) ctrl->ticks += 1u;
+    if (!a.rec || !a.n_slots) return;
+    __syncthreads();
+    publish_slots<kSweepThreads>(a, s_prefix, cur ^ 1u, hist);
 }
 
 // Serving admission: seeds of newly admitted instances, (x, y, slot) triples.  Queues the
@@ -1334,6 +1407,46 @@ static hipError_t launch_sweep_t(const SolveArgs& a, uint32_t grid, hipStream_t 
         hipLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST>), dim3(grid), dim3(kSweepThreads), 0,
                            s, a);
     return hipGetLastError();
+}
+
+template <int TW, int TH, int KIND>
+static hipError_t launch_fused_t(const SolveArgs& a, uint32_t grid, hipStream_t s, hipEvent_t ev0,
+                                 hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve_sweep_fused<TW, TH, KIND>), dim3(grid), dim3(kSweepThreads),
+                              0, s, ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL((k_solve_sweep_fused<TW, TH, KIND>), dim3(grid), dim3(kSweepThreads), 0,
+                           s, a);
+    return hipGetLastError();
+}
+
+template <int TW, int TH, int KIND>
+static int occ_fused_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep_fused<TW, TH, KIND>,
+                                                     kSweepThreads, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <int TW, int TH>
+static SolveKernel pick_fused(int kind) {
+    switch (kind) {
+        case 0: return {&launch_fused_t<TW, TH, 0>, &occ_fused_t<TW, TH, 0>};
+        case 1: return {&launch_fused_t<TW, TH, 1>, &occ_fused_t<TW, TH, 1>};
+        case 2: return {&launch_fused_t<TW, TH, 2>, &occ_fused_t<TW, TH, 2>};
+        case 3: return {&launch_fused_t<TW, TH, 3>, &occ_fused_t<TW, TH, 3>};
+    }
+    return {nullptr, nullptr};
+}
+
+SolveKernel get_fused_kernel(int tw, int th, int kind, int mode) {
+    if (mode != 2) return {nullptr, nullptr};
+    if (tw == 64 && th == 64) return pick_fused<64, 64>(kind);
+    if (tw == 64 && th == 32) return pick_fused<64, 32>(kind);
+    if (tw == 32 && th == 32) return pick_fused<32, 32>(kind);
+    return {nullptr, nullptr};
 }
 
 template <int TW, int TH, int KIND>

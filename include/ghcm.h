@@ -93,7 +93,9 @@ typedef struct ghcm_config {
                                        wave (default), 1 = one persistent cooperative
                                        launch runs every wave where built (sweep kernel;
                                        its grid barrier pays agent-scope fences),
-                                       2 = chunks of waves replayed as a HIP graph    */
+                                       2 = chunks of waves replayed as a HIP graph,
+                                       3 = one launch per wave: the tile-solve kernel's
+                                       last work-group runs the wave boundary (sweep) */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */

@@ -61,7 +61,8 @@ def test_c1_rings_every_config(c1, kernel, tile, sched):
 
 @pytest.mark.parametrize("tile", SWEEP_TILES)
 @pytest.mark.parametrize("sched,launch_mode", [(g.SCHED_DEVICE_TOPN, 0), (g.SCHED_DEVICE_TOPN, 1),
-                                               (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_HOST_PQ, 0)])
+                                               (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
+                                               (g.SCHED_HOST_PQ, 0)])
 @pytest.mark.parametrize("cap", [0, 1])
 def test_c1_rings_sweep_kernel(c1, tile, sched, launch_mode, cap):
     cost, seeds, ref32, ref64 = c1
