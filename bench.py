@@ -41,7 +41,7 @@ HBM_PEAK_GBS = 8000.0
 W = H = 4096
 FIELD_SEED, POINT_SEED = 20261202, 20261203
 METRIC = "Mcells/s at 4096² (% HBM peak); batched solves/s vs plain; p99 latency"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_sweep_c2_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_sweep_plain4096_traffic.txt")
 
 
 def log(*a):

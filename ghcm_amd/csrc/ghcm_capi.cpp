@@ -136,7 +136,7 @@ extern "C" int ghcm_device_info_get(int ordinal, ghcm_device_info* out) {
     HIP_TRY(hipGetDeviceProperties(&p, ordinal));
     std::memset(out, 0, sizeof *out);
     std::snprintf(out->name, sizeof out->name, "%s", p.name);
-    std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    std::snprintf(out->arch, sizeof out->arch, "%.*s", (int)sizeof out->arch - 1, p.gcnArchName);
     out->ordinal = ordinal;
     out->compute_units = p.multiProcessorCount;
     out->clock_khz = p.clockRate;
@@ -276,6 +276,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     for (uint32_t i = 0; i < n_inst; ++i) {
         const HostInst& x = hi[i];
         if (x.w == 0 || x.h == 0) return fail(GHCM_ERR_INVALID_ARG, "instance %u has zero size", i);
+        if (x.w > GHCM_MAX_WIDTH)
+            return fail(GHCM_ERR_INVALID_ARG, "instance %u width %u above %u", i, x.w, GHCM_MAX_WIDTH);
         if (!x.d_field || !x.d_time) return fail(GHCM_ERR_INVALID_ARG, "instance %u has a null buffer", i);
         if (x.n_seeds && !x.seeds) return fail(GHCM_ERR_INVALID_ARG, "instance %u seeds are null", i);
         for (uint32_t s = 0; s < x.n_seeds; ++s)

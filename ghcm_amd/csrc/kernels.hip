@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "ghcm_dev.h"
 #include "kernels.h"
 
@@ -36,16 +38,29 @@ namespace ghcm {
 
 enum { MODE_SYSTOLIC = 0, MODE_FIM = 1 };
 
+// Pointers read out of device memory (the instance records) are generic to the compiler,
+// which then emits flat accesses (64-bit lane addresses, both wait counters).  They always
+// address global memory; the cast lets it emit global loads and stores with a scalar base.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// Bytes per field element of a field kind (FP32 cost or speed, FP16 or BF16 speed).
 template <int KIND>
-__device__ __forceinline__ float load_cost(const void* f, size_t i) {
+constexpr uint32_t field_bytes() { return KIND <= 1 ? 4u : 2u; }
+
+// Cost of the field element at byte address p.
+template <int KIND>
+__device__ __forceinline__ float load_cost_at(const char* p) {
     if constexpr (KIND == 0) {
-        return static_cast<const float*>(f)[i];
+        return *gmem(reinterpret_cast<const float*>(p));
     } else if constexpr (KIND == 1) {
-        return 1.0f / static_cast<const float*>(f)[i];
+        return 1.0f / *gmem(reinterpret_cast<const float*>(p));
     } else if constexpr (KIND == 2) {
-        return 1.0f / static_cast<float>(static_cast<const _Float16*>(f)[i]);
+        return 1.0f / static_cast<float>(*gmem(reinterpret_cast<const _Float16*>(p)));
     } else {
-        uint32_t b = static_cast<uint32_t>(static_cast<const uint16_t*>(f)[i]) << 16;
+        uint32_t b = static_cast<uint32_t>(*gmem(reinterpret_cast<const uint16_t*>(p))) << 16;
         return 1.0f / __uint_as_float(b);
     }
 }
@@ -150,9 +165,6 @@ __device__ __forceinline__ uint32_t list_entry(const SolveArgs& a, const uint32_
     return a.list[((size_t)cur * kShards + lo) * a.shard_cap + (i - prefix[lo])];
 }
 
-// Stage a PW x PH box of time and cost (the tile plus its halo ring, origin (ox-1, oy-1))
-// into LDS; out-of-grid cells read +INF.  Every global load of a thread is issued before
-// the first LDS store, so the box costs one memory round trip rather than one per row.
 // Job j of the current wave: entry j of list[cur] when every queued tile was selected,
 // else entry j of the compacted top-n selection.
 __device__ __forceinline__ uint32_t job_tile(const SolveArgs& a, const uint32_t* prefix,
@@ -171,12 +183,52 @@ __device__ __forceinline__ uint32_t claim_tile(const SolveArgs& a, uint32_t g, u
     return mb;
 }
 
+// An instance record as wave-uniform register values.  The record is the same for every
+// lane of a work-group; passing each word through readfirstlane puts it in scalar registers,
+// so the tile's loads take a scalar base and carry no dependency on the vector load that
+// fetched the record (which would otherwise make every conditional staging load wait for
+// all loads before it).
+__device__ __forceinline__ GhcmInst uniform_inst(const GhcmInst& m) {
+    auto u64 = [](const void* p) {
+        const uint64_t v = reinterpret_cast<uint64_t>(p);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        return ((uint64_t)hi << 32) | lo;
+    };
+    GhcmInst r;
+    r.field = reinterpret_cast<const void*>(u64(m.field));
+    r.time = reinterpret_cast<float*>(u64(m.time));
+    r.w = __builtin_amdgcn_readfirstlane(m.w);
+    r.h = __builtin_amdgcn_readfirstlane(m.h);
+    r.tiles_x = __builtin_amdgcn_readfirstlane(m.tiles_x);
+    r.tiles_y = __builtin_amdgcn_readfirstlane(m.tiles_y);
+    r.tile_base = __builtin_amdgcn_readfirstlane(m.tile_base);
+    r.pad = 0u;
+    return r;
+}
+
+// The record of a single-instance solve, read once per work-group before its job loop.
+__device__ __forceinline__ GhcmInst single_inst(const SolveArgs& a) {
+    GhcmInst r{};
+    if (a.n_inst == 1) r = uniform_inst(a.inst[0]);
+    return r;
+}
+
+// Stage a PW x PH box of time and cost (the tile plus its halo ring, origin (ox-1, oy-1))
+// into LDS; out-of-grid cells read +INF.  Every global load of a thread is issued before
+// the first LDS store, so the box costs one memory round trip rather than one per row.
 template <int PW, int PH, int NT, int KIND>
 __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, int tid,
                                            float* t_s, float* c_s) {
     constexpr int PP = PW * PH;
     constexpr int NL = (PP + NT - 1) / NT;
     const int W = (int)in.w, H = (int)in.h;
+    // Addresses are a wave-uniform box origin plus a 32-bit byte offset per cell (the width
+    // limit of the C-ABI keeps PH rows of a grid below 2^32 bytes), so every load has a
+    // scalar base and one vector offset.
+    const int64_t org = (int64_t)(oy - 1) * W + (ox - 1);
+    const char* const tb = reinterpret_cast<const char*>(in.time) + org * 4;
+    const char* const fb = static_cast<const char*>(in.field) + org * (int64_t)field_bytes<KIND>();
     float tv[NL], cv[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
@@ -186,9 +238,9 @@ __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, i
         tv[k] = __builtin_inff();
         cv[k] = __builtin_inff();
         if (i < PP && gx >= 0 && gy >= 0 && gx < W && gy < H) {
-            const size_t idx = (size_t)gy * (size_t)W + (size_t)gx;
-            tv[k] = in.time[idx];
-            cv[k] = load_cost<KIND>(in.field, idx);
+            const uint32_t off = (uint32_t)by * (uint32_t)W + (uint32_t)bx;
+            tv[k] = *gmem(reinterpret_cast<const float*>(tb + off * 4u));
+            cv[k] = load_cost_at<KIND>(fb + off * field_bytes<KIND>());
         }
     }
 #pragma unroll
@@ -212,7 +264,7 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
     const int lane = threadIdx.x;
 
     const uint32_t ii = (a.n_inst == 1) ? 0u : a.tinst[g];
-    const GhcmInst in = a.inst[ii];
+    const GhcmInst in = uniform_inst(a.inst[ii]);
     const uint32_t lt = g - in.tile_base;
     const int tx = (int)(lt % in.tiles_x);
     const int ty = (int)(lt / in.tiles_x);
@@ -391,6 +443,7 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
     __syncthreads();
     // ---- commit improved cells (plain stores: this work-group is the only writer of
     // the tile's cells in this wave), tile min ----
+    char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
     float tmin = __builtin_inff();
     uint32_t improved = 0;
 #pragma unroll
@@ -402,7 +455,7 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
                 float tn = t_s[(y + 1) * PW + x + 1];
                 tmin = fminf(tmin, tn);
                 if (tn < told[k]) {
-                    in.time[(size_t)(oy + y) * (size_t)W + (size_t)(ox + x)] = tn;
+                    *gmem(reinterpret_cast<float*>(cb + ((uint32_t)y * (uint32_t)W + (uint32_t)x) * 4u)) = tn;
                     ++improved;
                 }
             }
@@ -522,7 +575,7 @@ constexpr int sweep_lds_words() { return 2 * (TW + 2) * (TH + 2) + 1; }
 template <int TW, int TH, int KIND, bool HOST>
 __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
                            uint32_t p, uint32_t job_slot, float* sh,
-                           uint32_t* s_u32, float* s_f32, const GhcmInst& in_single) {
+                           uint32_t* s_u32, float* s_f32, GhcmInst in_single) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
     constexpr int PP = PW * PH;
@@ -535,8 +588,11 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const int tid = threadIdx.x;
     const int w = tid >> 6, L = tid & 63;
 
-    // One instance: its record was loaded before the job loop (one dependent load fewer).
-    const GhcmInst in = (a.n_inst == 1) ? in_single : a.inst[a.tinst[g]];
+    // One instance: its record was read before the job loop (one dependent load fewer).
+    // It comes by value and is copied, not chosen by reference: a choice between two
+    // addresses would keep it in scratch behind flat loads.
+    GhcmInst in = in_single;
+    if (a.n_inst != 1) in = uniform_inst(a.inst[a.tinst[g]]);
     const uint32_t lt = g - in.tile_base;
     const int tx = (int)(lt % in.tiles_x);
     const int ty = (int)(lt / in.tiles_x);
@@ -705,6 +761,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     }
 
     // Commit improved cells (plain stores: one writer per tile per wave), tile min.
+    char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
     float tmin = __builtin_inff();
     uint32_t improved = 0;
 #pragma unroll
@@ -716,7 +773,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
                 const float tn = t_s[(cy + 1) * PW + cx + 1];
                 tmin = fminf(tmin, tn);
                 if (tn < told[k]) {
-                    in.time[(size_t)(oy + cy) * (size_t)W + (size_t)(ox + cx)] = tn;
+                    *gmem(reinterpret_cast<float*>(cb + ((uint32_t)cy * (uint32_t)W + (uint32_t)cx) * 4u)) = tn;
                     ++improved;
                 }
             }
@@ -761,8 +818,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
     __shared__ float s_f32[8];
     const int tid = threadIdx.x;
     // Single-instance solves: the instance record is read once, next to the control block.
-    GhcmInst in0{};
-    if (a.n_inst == 1) in0 = a.inst[0];
+    const GhcmInst in0 = single_inst(a);
     if constexpr (HOST) {
         const uint32_t j = blockIdx.x;
         if (j >= a.n_host_jobs) return;
@@ -878,76 +934,84 @@ __device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint
         }
         __syncthreads();
     }
-    auto entry_g = [&](uint32_t i) { return staged ? sg[i] : list_entry(a, s_prefix, cur, i); };
-    auto entry_k = [&](uint32_t i, uint32_t g) { return staged ? sk[i] : key[g]; };
-    uint32_t lo_base = 0u, below = 0u;
-    constexpr int BPT = 4096 / NT;                    // bins per thread
-    for (int level = 0; level < 2; ++level) {
-        const int shift = level == 0 ? 19 : 7;
-        for (int i = tid; i < 4096; i += NT) hist[i] = 0u;
-        __syncthreads();
-        for (uint32_t i = tid; i < c; i += NT) {
-            const uint32_t k = entry_k(i, staged ? 0u : list_entry(a, s_prefix, cur, i));
-            if (level == 1 && (k >> 19) != (lo_base >> 19)) continue;
-            atomicAdd(&hist[(k >> shift) & 0xFFFu], 1u);
-        }
-        __syncthreads();
-        uint32_t mine = 0;
-        for (int j = 0; j < BPT; ++j) mine += hist[BPT * tid + j];
-        uint32_t tot;
-        const uint32_t excl = block_excl_scan<NT>(mine, part, &tot) + below;
-        if (excl < n && excl + mine >= n) {           // the crossing bin is one of mine
-            uint32_t cum = excl, b = BPT * tid;
-            for (int j = 0; j < BPT; ++j) {
-                const uint32_t h = hist[BPT * tid + j];
-                if (cum + h >= n) { b = BPT * tid + j; break; }
-                cum += h;
+    // Two copies of the histogram and compaction passes, one reading the LDS stage and one
+    // the list in HBM: a runtime choice between the two would make every read a flat one.
+    uint32_t base = 0, carried = 0;
+    auto passes = [&](auto staged_c) {
+        constexpr bool S = decltype(staged_c)::value;
+        auto entry_g = [&](uint32_t i) { return S ? sg[i] : list_entry(a, s_prefix, cur, i); };
+        auto entry_k = [&](uint32_t i, uint32_t g) { return S ? sk[i] : key[g]; };
+        uint32_t lo_base = 0u, below = 0u;
+        constexpr int BPT = 4096 / NT;                    // bins per thread
+        for (int level = 0; level < 2; ++level) {
+            const int shift = level == 0 ? 19 : 7;
+            for (int i = tid; i < 4096; i += NT) hist[i] = 0u;
+            __syncthreads();
+            for (uint32_t i = tid; i < c; i += NT) {
+                const uint32_t k = entry_k(i, S ? 0u : list_entry(a, s_prefix, cur, i));
+                if (level == 1 && (k >> 19) != (lo_base >> 19)) continue;
+                atomicAdd(&hist[(k >> shift) & 0xFFFu], 1u);
             }
-            s_misc[0] = b;
-            s_misc[1] = cum;
-        }
-        __syncthreads();
-        lo_base |= s_misc[0] << shift;
-        below = s_misc[1];
-        __syncthreads();
-    }
-    // Compaction: keys < lo_base (there are `below` of them) all go; keys in
-    // [lo_base, lo_base + 128) go while the quota n - below lasts.
-    const uint32_t quota = n - below;
-    uint32_t base = 0, quota_used = 0, carried = 0;
-    const uint32_t thr_hi = lo_base + (1u << 7);
-    for (uint32_t i0 = 0; i0 < c; i0 += NT) {
-        const uint32_t i = i0 + tid;
-        uint32_t g = 0, k = 0xFFFFFFFFu, low = 0, tie = 0;
-        if (i < c) {
-            g = entry_g(i);
-            k = entry_k(i, g);
-            low = k < lo_base ? 1u : 0u;
-            tie = (!low && k < thr_hi) ? 1u : 0u;
-        }
-        uint32_t tot_tie;
-        const uint32_t tie_rank = block_excl_scan<NT>(tie, part, &tot_tie) + quota_used;
-        const uint32_t sel = low | ((tie && tie_rank < quota) ? 1u : 0u);
-        uint32_t tot_sel;
-        const uint32_t pos = block_excl_scan<NT>(sel, part, &tot_sel);
-        if (i < c) {
-            if (sel) {
-                a.jobs[base + pos] = g;
-            } else {
-                // carry: move the activation record to the next parity, re-queue
-                uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
-                uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
-                const uint32_t mb = *mp;
-                const uint32_t kk = *kp;
-                *kp = kInfBits;
-                *mp = 0u;
-                activate(a, g, wave, cur, p, mb, __uint_as_float(kk));
+            __syncthreads();
+            uint32_t mine = 0;
+            for (int j = 0; j < BPT; ++j) mine += hist[BPT * tid + j];
+            uint32_t tot;
+            const uint32_t excl = block_excl_scan<NT>(mine, part, &tot) + below;
+            if (excl < n && excl + mine >= n) {           // the crossing bin is one of mine
+                uint32_t cum = excl, b = BPT * tid;
+                for (int j = 0; j < BPT; ++j) {
+                    const uint32_t h = hist[BPT * tid + j];
+                    if (cum + h >= n) { b = BPT * tid + j; break; }
+                    cum += h;
+                }
+                s_misc[0] = b;
+                s_misc[1] = cum;
             }
+            __syncthreads();
+            lo_base |= s_misc[0] << shift;
+            below = s_misc[1];
+            __syncthreads();
         }
-        base += tot_sel;
-        quota_used += tot_tie;
-        carried += (i0 + NT <= c ? NT : c - i0) - tot_sel;
-    }
+        // Compaction: keys < lo_base (there are `below` of them) all go; keys in
+        // [lo_base, lo_base + 128) go while the quota n - below lasts.
+        const uint32_t quota = n - below;
+        uint32_t quota_used = 0;
+        const uint32_t thr_hi = lo_base + (1u << 7);
+        for (uint32_t i0 = 0; i0 < c; i0 += NT) {
+            const uint32_t i = i0 + tid;
+            uint32_t g = 0, k = 0xFFFFFFFFu, low = 0, tie = 0;
+            if (i < c) {
+                g = entry_g(i);
+                k = entry_k(i, g);
+                low = k < lo_base ? 1u : 0u;
+                tie = (!low && k < thr_hi) ? 1u : 0u;
+            }
+            uint32_t tot_tie;
+            const uint32_t tie_rank = block_excl_scan<NT>(tie, part, &tot_tie) + quota_used;
+            const uint32_t sel = low | ((tie && tie_rank < quota) ? 1u : 0u);
+            uint32_t tot_sel;
+            const uint32_t pos = block_excl_scan<NT>(sel, part, &tot_sel);
+            if (i < c) {
+                if (sel) {
+                    a.jobs[base + pos] = g;
+                } else {
+                    // carry: move the activation record to the next parity, re-queue
+                    uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
+                    uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
+                    const uint32_t mb = *mp;
+                    const uint32_t kk = *kp;
+                    *kp = kInfBits;
+                    *mp = 0u;
+                    activate(a, g, wave, cur, p, mb, __uint_as_float(kk));
+                }
+            }
+            base += tot_sel;
+            quota_used += tot_tie;
+            carried += (i0 + NT <= c ? NT : c - i0) - tot_sel;
+        }
+    };
+    if (staged) passes(std::true_type{});
+    else passes(std::false_type{});
     if (tid == 0) {
         ctrl->jobs_from_list = 0u;
         ctrl->n_jobs = base;
@@ -1029,7 +1093,7 @@ __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_ins
             a.tstat[2 * T + g] = 0u;
             if (a.tinst) const_cast<uint32_t*>(a.tinst)[g] = j;
         }
-        uint32_t* tm = reinterpret_cast<uint32_t*>(in.time);
+        auto* tm = gmem(reinterpret_cast<uint32_t*>(in.time));
         const size_t cells = (size_t)in.w * in.h;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += stride)
             tm[i] = kInfBits;
@@ -1089,8 +1153,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
     __shared__ uint32_t s_prefix[kShards + 1];
     __shared__ uint32_t s_misc[4];
     const int tid = threadIdx.x;
-    GhcmInst in0{};
-    if (a.n_inst == 1) in0 = a.inst[0];
+    const GhcmInst in0 = single_inst(a);
     GhcmCtrl* ctrl = a.ctrl;
     publish_progress(a, ctrl);
     const uint32_t done = ctrl->done;
@@ -1148,7 +1211,7 @@ __global__ void k_seed_serve(SolveArgs a, const uint32_t* seeds, uint32_t n_seed
     const uint32_t np = (wave + 1u) & 1u;
     const uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], ii = seeds[3 * s + 2];
     const GhcmInst in = a.inst[ii];
-    in.time[(size_t)y * in.w + x] = 0.0f;
+    gmem(in.time)[(size_t)y * in.w + x] = 0.0f;
     const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
     for (int d = 0; d < 5; ++d) {
         const int px = (int)x + dx[d], py = (int)y + dy[d];
@@ -1213,8 +1276,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
     uint32_t* hist = reinterpret_cast<uint32_t*>(sh);
     uint32_t* part = reinterpret_cast<uint32_t*>(sh) + 4096;
     GhcmCtrl* ctrl = a.ctrl;
-    GhcmInst in0{};
-    if (a.n_inst == 1) in0 = a.inst[0];
+    const GhcmInst in0 = single_inst(a);
     const int tid = threadIdx.x;
     const uint32_t G = gridDim.x;
     const uint32_t n_groups = (G + kBarGroup - 1) / kBarGroup;
@@ -1323,7 +1385,7 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
     if (s >= n_seeds) return;
     const uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], ii = seeds[3 * s + 2];
     const GhcmInst in = a.inst[ii];
-    in.time[(size_t)y * in.w + x] = 0.0f;
+    gmem(in.time)[(size_t)y * in.w + x] = 0.0f;
     const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
     for (int d = 0; d < 5; ++d) {
         int px = (int)x + dx[d], py = (int)y + dy[d];

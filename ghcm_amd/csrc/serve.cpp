@@ -714,6 +714,7 @@ namespace {
 int check_request(ghcm_server* s, uint32_t w, uint32_t h, const uint32_t* seeds_xy,
                   uint32_t n_seeds, bool device, Request* r) {
     if (w == 0 || h == 0) return fail(GHCM_ERR_INVALID_ARG, "zero-sized field");
+    if (w > GHCM_MAX_WIDTH) return fail(GHCM_ERR_INVALID_ARG, "width %u above %u", w, GHCM_MAX_WIDTH);
     if (n_seeds && !seeds_xy) return fail(GHCM_ERR_INVALID_ARG, "seeds are null");
     for (uint32_t k = 0; k < n_seeds; ++k)
         if (seeds_xy[2 * k] >= w || seeds_xy[2 * k + 1] >= h)

@@ -26,6 +26,10 @@ extern "C" {
 
 #define GHCM_ABI_VERSION 1
 
+/* Widest grid a solve accepts (cells per row).  Kernels address a tile and its halo as a
+ * 32-bit byte offset from the tile's origin, so rows of a tile must span less than 4 GiB. */
+#define GHCM_MAX_WIDTH 4194304u
+
 /* Status codes.  Replaces cl::Error + cl_do's exit(-1) (src/util_opencl.h:46-56). */
 typedef enum ghcm_status {
     GHCM_OK                  = 0,

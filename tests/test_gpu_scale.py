@@ -10,20 +10,20 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c2():
+def plain4096():
     speed = g.gen_smooth_speed(4096, 4096, 20261202, 32.0, 0.1, 10.0)
     seeds = g.gen_points(4096, 4096, 20261203, 4)
     cost = g.cost_from_speed(speed)
     return speed, cost, seeds
 
 
-def test_c2_full_size_parity(c2):
-    speed, cost, seeds = c2
+def test_plain4096_full_size_parity(plain4096):
+    speed, cost, seeds = plain4096
     ref = oracle.fmm_f32(cost, seeds)
     for tile, kernel, lm in (((16, 16), g.KERNEL_SYSTOLIC, 0), ((8, 8), g.KERNEL_FIM, 0),
                              ((64, 64), g.KERNEL_SWEEP, 0), ((64, 64), g.KERNEL_SWEEP, 1),
                              ((64, 64), g.KERNEL_SWEEP, 3)):
-        print(f"case c2 tile={tile} kernel={kernel} launch_mode={lm}", flush=True)
+        print(f"case plain4096 tile={tile} kernel={kernel} launch_mode={lm}", flush=True)
         with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=512,
                        inner_iter_cap=1 if kernel == g.KERNEL_SWEEP else 0, launch_mode=lm) as ctx:
             dc = ctx.to_device(cost)
@@ -167,13 +167,13 @@ def contrast_4096():
 
 
 @pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
-def test_c3_contrast_4096_vs_fp64(contrast_4096, kind, tol):
+def test_contrast_4096_vs_fp64(contrast_4096, kind, tol):
     """1e-4..1 speed contrast at full size: every storage precision against the FP64 FMM."""
     speed, seeds, ref64 = contrast_4096
     field = {"speed_f32": lambda: speed.astype(np.float32),
              "speed_f16": lambda: speed.astype(np.float16),
              "speed_bf16": lambda: g.to_bf16(speed.astype(np.float32))}[kind]()
-    print(f"case c3 contrast 4096 {kind}", flush=True)
+    print(f"case contrast 4096 {kind}", flush=True)
     with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
                    inner_iter_cap=1) as ctx:
         t, st = ctx.solve(field, seeds, kind=g.KINDS[kind])
@@ -184,13 +184,13 @@ def test_c3_contrast_4096_vs_fp64(contrast_4096, kind, tol):
     assert m["linf"] <= tol * m["max_ref"]
 
 
-def test_c3_fp32_vs_cpu_fim(c2):
+def test_plain4096_fp32_vs_cpu_fim(plain4096):
     """The 4096^2 smooth field: the HIP path against the oracle FP32 FIM (the reference's
     second CPU solver), and both against the FP64 FMM."""
-    speed, cost, seeds = c2
+    speed, cost, seeds = plain4096
     ref_fim, _ = oracle.fim_f32(cost, seeds, 1e-5)
     ref64 = oracle.fmm_f64(1.0 / speed, seeds)
-    print("case c3 fp32 vs fim", flush=True)
+    print("case plain4096 fp32 vs fim", flush=True)
     with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
                    inner_iter_cap=1) as ctx:
         t, _ = ctx.solve(cost, seeds)
