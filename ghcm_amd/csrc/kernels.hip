@@ -65,34 +65,32 @@ __device__ __forceinline__ float load_cost_at(const char* p) {
     }
 }
 
-// Godunov upwind update for |grad T| = c with h = 1, cancellation-safe form.
-// m = min(a,b), d = |a-b|; if c < d the 1-D update m + c applies (data/eikonal_common.cl:
-// 296-297); otherwise T = m + (d + sqrt((sqrt2 c - d)(sqrt2 c + d)))/2, the root of
-// (T-a)^2 + (T-b)^2 = c^2 (:299-300) with the discriminant factored and the large
-// common part m added once.  Result min'ed with t (:305-308).
-// Branch-free: both candidates are formed and one is selected.  `!(c >= d)` also takes
-// the 1-D branch when d is NaN (a = b = +INF), which yields +INF.  The square root is the
-// hardware v_sqrt_f32 (1 ulp); its argument lies in [c^2, 2c^2].
-__device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
-    const float m = fminf(ax, ay);
-    const float M = fmaxf(ax, ay);
-    const float d = M - m;
-    const float s = 1.41421356237309515f * c;
-    float q = (s - d) * (s + d);
-    q = q > 0.0f ? q : 0.0f;
-    const float two = m + 0.5f * (d + __builtin_amdgcn_sqrtf(q));
-    const float r = (c >= d) ? two : m + c;
-    return fminf(t, r);
-}
-
-// The same update without the final min against the old time, for callers that compare
-// against it themselves; m = min(ax, ay) is passed in, |ax - ay| takes the place of max - min.
+// Godunov upwind update for |grad T| = c with h = 1, cancellation-safe form, without the
+// final min against the old time.  m = min(a,b) is passed in and d = |a-b|; if c < d the 1-D
+// update m + c applies (data/eikonal_common.cl:296-297); otherwise T = m + d/2 +
+// sqrt((sqrt2 c - d)(sqrt2 c + d))/2, the root of (T-a)^2 + (T-b)^2 = c^2 (:299-300) with
+// the discriminant factored and the large common part m added outside the root.
+// Branch-free: both candidates are formed and one is selected.  `!(c >= d)` also takes the
+// 1-D branch when d is NaN (a = b = +INF), which yields +INF.  The square root is the
+// hardware v_sqrt_f32 (1 ulp); its argument lies in [c^2, 2c^2] on the selected branch.
+// The large common part m is added by one final rounding of m + (d + root)/2, the
+// parenthesis formed at the scale of c: the result then never rounds below max(a,b) where
+// the exact root equals it (c = d), so a cell never drops below a neighbour it was computed
+// from.  (Rounding m + d/2 first and adding root/2 after breaks that: two cells can then
+// lower each other by an ulp at a time without end.)
+// Every update of a cell and every border prediction of a neighbour's update goes through
+// this one function.
 __device__ __forceinline__ float godunov_raw(float ax, float ay, float m, float c) {
     const float d = fabsf(ax - ay);                // NaN when both are +INF
     const float s = 1.41421356237309515f * c;
     // (s - d)(s + d) >= 0 whenever c >= d; the other branch discards it.
     const float two = 0.5f * (d + __builtin_amdgcn_sqrtf((s - d) * (s + d)));
     return m + ((c >= d) ? two : c);
+}
+
+// The update min'ed with the old time t (:305-308).
+__device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
+    return fminf(t, godunov_raw(ax, ay, fminf(ax, ay), c));
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -525,14 +523,18 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
         publish_progress(a, ctrl);
-        if (ctrl->done) return;
+        // The control words and the shard prefix are read in one round trip; the done check
+        // follows.
+        const uint32_t done = ctrl->done;
         const uint32_t nj = ctrl->n_jobs;
-        if (blockIdx.x >= nj) return;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
+        static_assert(kShards < 64, "one lane per prefix entry");
+        const uint32_t pv = lane <= (int)kShards ? ctrl->prefix[lane] : 0u;
+        if (done || blockIdx.x >= nj) return;
         const uint32_t p = wave & 1u;
-        for (int i = lane; i <= (int)kShards; i += 64) s_prefix[i] = ctrl->prefix[i];
+        if (lane <= (int)kShards) s_prefix[lane] = pv;
         __syncthreads();
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
@@ -827,15 +829,17 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
         publish_progress(a, ctrl);
-        // All control words are read together; the done check follows.
+        // The control words and the shard prefix are read in one round trip; the done check
+        // follows.
         const uint32_t done = ctrl->done;
         const uint32_t nj = ctrl->n_jobs;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
         if (done || blockIdx.x >= nj) return;
         const uint32_t p = wave & 1u;
-        for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
+        if (tid <= (int)kShards) s_prefix[tid] = pv;
         __syncthreads();
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
@@ -885,30 +889,43 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, 
 // then [18:7] (keys are non-negative floats, so their bit patterns order like the values)
 // gives the threshold bin; entries below it and, up to the quota, inside it are compacted
 // into jobs[]; the rest are carried to the next list with their key and border bits.
+// Returns the new current list, or -1 without writing anything when `stop_if_done` is set
+// and the loop is already done.
 template <int NT, int STAGE>
-__device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint32_t* part,
-                            uint32_t* s_prefix, uint32_t* s_misc, uint32_t* sg, uint32_t* sk) {
+__device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint32_t* part,
+                           uint32_t* s_prefix, uint32_t* s_misc, uint32_t* sg, uint32_t* sk,
+                           bool stop_if_done = false) {
+    static_assert(kShards == 32 && NT >= 64, "one wave64 holds the shard counts of both lists");
     GhcmCtrl* ctrl = a.ctrl;
     const int tid = threadIdx.x;
-    uint32_t cur = ctrl->cur;
+    // The control words and the shard counts of both lists in one round trip.
+    const uint32_t done = stop_if_done ? ctrl->done : 0u;
+    const uint32_t cur0 = ctrl->cur;
     const uint32_t wave0 = ctrl->wave;
-    __syncthreads();
-    if (!first) {
-        // list[cur] was consumed by the wave that just ran: empty it and swap.
-        if (tid < (int)kShards) ctrl->count[cur][tid * kCounterStride] = 0u;
-        cur ^= 1u;
-    }
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t sh = 0; sh < kShards; ++sh) {
-            s_prefix[sh] = acc;
-            acc += ctrl->count[cur][sh * kCounterStride];
+    const uint32_t n = ctrl->n_select;
+    uint32_t cnt = 0u;
+    if (tid < 2 * (int)kShards) cnt = ctrl->count[tid >> 5][(tid & 31) * kCounterStride];
+    if (done) return -1;
+    // list[cur0] was consumed by the wave that just ran (unless `first`): it is emptied and
+    // the other list becomes current.
+    const uint32_t cur = first ? cur0 : (cur0 ^ 1u);
+    if (tid < 64) {
+        // Lanes 0-31 scan list 0's counts, lanes 32-63 list 1's.
+        const uint32_t sl = (uint32_t)tid & 31u, lst = (uint32_t)tid >> 5;
+        uint32_t x = cnt;
+#pragma unroll
+        for (uint32_t off = 1; off < kShards; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (sl >= off) x += y;
         }
-        s_prefix[kShards] = acc;
+        if (lst == cur) {
+            s_prefix[sl + 1] = x;
+            if (sl == 0) s_prefix[0] = 0u;
+        }
+        if (!first && lst == cur0) ctrl->count[cur0][sl * kCounterStride] = 0u;
     }
     __syncthreads();
     const uint32_t c = s_prefix[kShards];
-    const uint32_t n = ctrl->n_select;
     const uint32_t wave = wave0 + (first ? 0u : 1u);
     const uint32_t p = wave & 1u;
     if (tid <= (int)kShards) ctrl->prefix[tid] = s_prefix[tid];
@@ -921,7 +938,7 @@ __device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint
         ctrl->jobs_from_list = 1u;
         ctrl->n_jobs = c;
     }
-    if (c <= n) return;
+    if (c <= n) return (int)cur;
 
     const uint32_t* key = a.key + (size_t)p * a.tiles;
     // Entries (tile, key) are read from the list once into LDS when they fit.
@@ -1017,6 +1034,7 @@ __device__ void wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint
         ctrl->n_jobs = base;
         ctrl->carried += carried;
     }
+    return (int)cur;
 }
 
 constexpr int kSelectStage = 4096;     // list entries staged in LDS by k_wave_end
@@ -1047,17 +1065,13 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     __shared__ uint32_t sg[kSelectStage];
     __shared__ uint32_t sk[kSelectStage];
     const int tid = threadIdx.x;
-    if (a.ctrl->done) {
-        // Idle launch: only the tick advances, so the host still sees progress.
-        if (tid == 0) a.ctrl->ticks += 1u;
-        return;
-    }
-    const uint32_t cur0 = a.ctrl->cur;
-    wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc, sg, sk);
+    const int cur = wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc,
+                                                    sg, sk, true);
+    // An idle launch (the loop is done) only advances the tick, so the host still sees progress.
     if (tid == 0) a.ctrl->ticks += 1u;
-    if (!a.rec || !a.n_slots) return;
+    if (cur < 0 || !a.rec || !a.n_slots) return;
     __syncthreads();
-    publish_slots<1024>(a, s_prefix, first ? cur0 : (cur0 ^ 1u), hist);
+    publish_slots<1024>(a, s_prefix, (uint32_t)cur, hist);
 }
 
 // Per-solve initialisation in one launch (grid.y over instances): the control block and
