@@ -125,6 +125,7 @@ EXPORTED_SYMBOLS = (
     "ghcm_abi_version", "ghcm_status_string", "ghcm_last_error", "ghcm_config_default",
     "ghcm_device_count", "ghcm_device_info_get", "ghcm_ctx_create", "ghcm_ctx_destroy",
     "ghcm_ctx_get_config", "ghcm_ctx_set_config", "ghcm_solve", "ghcm_solve_device",
+    "ghcm_solve3d", "ghcm_solve3d_device",
     "ghcm_dev_alloc", "ghcm_dev_free", "ghcm_dev_upload", "ghcm_dev_download", "ghcm_dev_copy",
     "ghcm_solve_batch", "ghcm_tune_default_space", "ghcm_autotune",
     "ghcm_serve_config_default", "ghcm_serve_start", "ghcm_serve_submit",
@@ -171,6 +172,8 @@ def lib() -> ctypes.CDLL:
     L.ghcm_ctx_set_config.argtypes = [vp, P(Config)]
     L.ghcm_solve.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_device.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
+    L.ghcm_solve3d.argtypes = [vp, vp, i32, u32, u32, u32, vp, u32, vp, P(Stats)]
+    L.ghcm_solve3d_device.argtypes = [vp, vp, i32, u32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_batch.argtypes = [vp, i32, P(Instance), u32, P(Stats)]
     L.ghcm_tune_default_space.argtypes = [i32, P(TuneCandidate), u32, P(u32)]
     L.ghcm_autotune.argtypes = [vp, i32, P(Instance), u32, P(TuneCandidate), u32, u32, P(u32)]
@@ -341,6 +344,33 @@ class Context:
         if not (allow_not_converged and rc == ERR_NOT_CONVERGED):
             _check(rc)
         return out, st.to_dict()
+
+    def solve3d(self, field: np.ndarray, seeds, kind: int = FIELD_COST_F32,
+                allow_not_converged: bool = False):
+        """Solve one 3-D field (D, H, W) held in host memory; seeds are (x, y, z).
+        Returns (times (D, H, W) float32, stats)."""
+        f = _field_array(field, kind)
+        if f.ndim != 3:
+            raise ValueError("field must be 3-D (D, H, W)")
+        d, h, w = f.shape
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32).reshape(-1, 3))
+        out = np.empty((d, h, w), dtype=np.float32)
+        st = Stats()
+        rc = self._L.ghcm_solve3d(self._h, f.ctypes.data, kind, w, h, d, s.ctypes.data, s.shape[0],
+                                  out.ctypes.data, ctypes.byref(st))
+        if not (allow_not_converged and rc == ERR_NOT_CONVERGED):
+            _check(rc)
+        return out, st.to_dict()
+
+    def solve3d_device(self, d_field: int, kind: int, w: int, h: int, d: int, seeds,
+                       d_time: int) -> dict:
+        """Solve a 3-D field already resident in device memory (raw device pointers)."""
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32).reshape(-1, 3))
+        st = Stats()
+        _check(self._L.ghcm_solve3d_device(self._h, ctypes.c_void_p(d_field), kind, w, h, d,
+                                           s.ctypes.data, s.shape[0], ctypes.c_void_p(d_time),
+                                           ctypes.byref(st)))
+        return st.to_dict()
 
     def alloc(self, shape, dtype) -> "DeviceArray":
         """Device buffer on this context's device (freed with the object)."""

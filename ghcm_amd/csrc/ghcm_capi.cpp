@@ -30,6 +30,7 @@ using namespace ghcm;
 using ghcm_capi::DevBuf;
 using ghcm_capi::PinnedBuf;
 using ghcm_capi::bytes_per_visit;
+using ghcm_capi::bytes_per_visit3d;
 using ghcm_capi::fail;
 using ghcm_capi::kind_bytes;
 using ghcm_capi::valid_kind;
@@ -250,23 +251,33 @@ struct HostInst {      // host view of one instance
     const void* d_field;
     float* d_time;
     uint32_t w, h;
-    const uint32_t* seeds;
+    const uint32_t* seeds;  // (x, y) pairs, or (x, y, z) triples for a 3-D solve
     uint32_t n_seeds;
+    uint32_t d = 1;         // depth of a 3-D instance
 };
 
 
-int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats) {
+// One device wave loop over n_inst instances.  three_d: every instance is a W x H x D grid
+// solved in 8x8x8 tiles by the 3-D tile kernel (device scheduler, per-wave launches or
+// graph replay); the configuration's tile shape and in-tile style do not apply.
+int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
+               bool three_d = false) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     const ghcm_config& cfg = ctx->cfg;
     if (!valid_kind(kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", kind);
     if (n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
-    const uint32_t tw = cfg.tile_w, th = cfg.tile_h;
+    const uint32_t tw = three_d ? kTile3D : cfg.tile_w, th = three_d ? kTile3D : cfg.tile_h;
+    const uint32_t td = three_d ? kTile3D : 1u;
     const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
-    const SolveKernel sk = get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched);
+    if (three_d && host_sched)
+        return fail(GHCM_ERR_UNSUPPORTED, "3-D solves run with the device scheduler only");
+    const SolveKernel sk = three_d ? get_solve3d_kernel(kind)
+                                   : get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched);
     const solve_launcher_t launch = sk.launch;
     if (!launch)
         return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u kind %d kernel %d%s", tw, th,
                     kind, cfg.kernel, host_sched ? " (host scheduler)" : "");
+    const uint32_t sd = three_d ? 3u : 2u;            // seed components
 
     // ---- instance table, seeds ----
     int rc = ctx->h_inst.reserve(sizeof(GhcmInst) * n_inst);
@@ -280,10 +291,14 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
             return fail(GHCM_ERR_INVALID_ARG, "instance %u width %u above %u", i, x.w, GHCM_MAX_WIDTH);
         if (!x.d_field || !x.d_time) return fail(GHCM_ERR_INVALID_ARG, "instance %u has a null buffer", i);
         if (x.n_seeds && !x.seeds) return fail(GHCM_ERR_INVALID_ARG, "instance %u seeds are null", i);
-        for (uint32_t s = 0; s < x.n_seeds; ++s)
-            if (x.seeds[2 * s] >= x.w || x.seeds[2 * s + 1] >= x.h)
-                return fail(GHCM_ERR_INVALID_ARG, "instance %u seed %u (%u,%u) outside %ux%u", i, s,
-                            x.seeds[2 * s], x.seeds[2 * s + 1], x.w, x.h);
+        const uint32_t dd = three_d ? x.d : 1u;
+        if (dd == 0) return fail(GHCM_ERR_INVALID_ARG, "instance %u has zero depth", i);
+        for (uint32_t s = 0; s < x.n_seeds; ++s) {
+            const uint32_t sx = x.seeds[sd * s], sy = x.seeds[sd * s + 1], sz = three_d ? x.seeds[sd * s + 2] : 0u;
+            if (sx >= x.w || sy >= x.h || sz >= dd)
+                return fail(GHCM_ERR_INVALID_ARG, "instance %u seed %u (%u,%u,%u) outside %ux%ux%u", i, s,
+                            sx, sy, sz, x.w, x.h, dd);
+        }
         GhcmInst& g = hinst[i];
         g.field = x.d_field;
         g.time = x.d_time;
@@ -292,9 +307,12 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         g.tiles_x = (x.w + tw - 1) / tw;
         g.tiles_y = (x.h + th - 1) / th;
         g.tile_base = (uint32_t)tiles64;
+        g.d = dd;
+        g.tiles_z = (dd + td - 1) / td;
         g.pad = 0;
-        tiles64 += (uint64_t)g.tiles_x * g.tiles_y;
-        cells += (uint64_t)x.w * x.h;
+        tiles64 += (uint64_t)g.tiles_x * g.tiles_y * g.tiles_z;
+        if (tiles64 >= 0x7FFFFFFFull) return fail(GHCM_ERR_INVALID_ARG, "too many tiles");
+        cells += (uint64_t)x.w * x.h * dd;
         n_seeds_total += x.n_seeds;
     }
     if (tiles64 >= 0x7FFFFFFFull) return fail(GHCM_ERR_INVALID_ARG, "too many tiles");
@@ -313,7 +331,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
     if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
     if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
-    if ((rc = ctx->h_seeds.reserve(12ull * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
+    if ((rc = ctx->h_seeds.reserve(4ull * (sd + 1) * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
     if ((rc = ctx->h_ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
     if ((rc = ctx->rec.reserve(sizeof(HostRec)))) return rc;
 
@@ -322,8 +340,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         size_t k = 0;
         for (uint32_t i = 0; i < n_inst; ++i)
             for (uint32_t s = 0; s < hi[i].n_seeds; ++s) {
-                hs[k++] = hi[i].seeds[2 * s];
-                hs[k++] = hi[i].seeds[2 * s + 1];
+                for (uint32_t c = 0; c < sd; ++c) hs[k++] = hi[i].seeds[sd * s + c];
                 hs[k++] = i;
             }
     }
@@ -335,7 +352,8 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
     const uint32_t grid = std::max(1u, std::min(n_sel, resident));
     uint64_t max_cells = 0;
-    for (uint32_t i = 0; i < n_inst; ++i) max_cells = std::max<uint64_t>(max_cells, (uint64_t)hi[i].w * hi[i].h);
+    for (uint32_t i = 0; i < n_inst; ++i)
+        max_cells = std::max<uint64_t>(max_cells, (uint64_t)hi[i].w * hi[i].h * (three_d ? hi[i].d : 1u));
     volatile HostRec* hr = ctx->rec.as<HostRec>();
     hr->progress = 0ull;
 
@@ -361,7 +379,10 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
     HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel,
                               ctx->totals.as<unsigned long long>(), st));
-    HIP_TRY(launch_seed(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
+    if (three_d)
+        HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, st));
+    else
+        HIP_TRY(launch_seed(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
 
     uint64_t max_waves = cfg.max_waves ? cfg.max_waves : (uint64_t)8 * T + 100000;
     // Per-tile counters are summed once after the last wave (outside the timed span) and
@@ -392,9 +413,9 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
 
     // launch_mode 3: tile solve and wave boundary in one launch (sweep kernel).
     const SolveKernel fk = get_fused_kernel((int)tw, (int)th, kind, cfg.kernel);
-    const bool fused = !host_sched && cfg.launch_mode == 3 && fk.launch != nullptr;
+    const bool fused = !three_d && !host_sched && cfg.launch_mode == 3 && fk.launch != nullptr;
     const PersistentKernel pk = get_persistent_kernel((int)tw, (int)th, kind, cfg.kernel);
-    const bool persistent = !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
+    const bool persistent = !three_d && !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
     if (persistent) {
         // ---- one cooperative launch runs every wave (grid = one resident round) ----
         const int per_cu = pk.blocks_per_cu();
@@ -652,7 +673,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         stats->tile_w = tw;
         stats->tile_h = th;
         stats->n_active = n_sel;
-        stats->bytes_per_visit = bytes_per_visit(tw, th, kind);
+        stats->bytes_per_visit = three_d ? bytes_per_visit3d(kTile3D, kind) : bytes_per_visit(tw, th, kind);
         stats->solve_kernel_us = kern_us;
         stats->solve_launches = kern_launches;
     }
@@ -690,6 +711,39 @@ extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint
     HIP_TRY(hipMemcpyAsync(ctx->field_io.p, field, n * kind_bytes(field_kind), hipMemcpyHostToDevice, ctx->stream));
     HostInst hi{ctx->field_io.p, ctx->time_io.as<float>(), w, h, seeds_xy, n_seeds};
     rc = solve_core(ctx, field_kind, &hi, 1, stats);
+    if (rc != GHCM_OK && rc != GHCM_ERR_NOT_CONVERGED) return rc;
+    HIP_TRY(hipMemcpyAsync(time_out, ctx->time_io.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->duration_total_us =
+            std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+extern "C" int ghcm_solve3d_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
+                                   uint32_t h, uint32_t d, const uint32_t* seeds_xyz, uint32_t n_seeds,
+                                   float* d_time, ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    HostInst hi{d_field, d_time, w, h, seeds_xyz, n_seeds, d};
+    return solve_core(ctx, field_kind, &hi, 1, stats, true);
+}
+
+extern "C" int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
+                            uint32_t d, const uint32_t* seeds_xyz, uint32_t n_seeds, float* time_out,
+                            ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    if (!field || !time_out) return fail(GHCM_ERR_INVALID_ARG, "null field or output");
+    if (!valid_kind(field_kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", field_kind);
+    if (w == 0 || h == 0 || d == 0) return fail(GHCM_ERR_INVALID_ARG, "zero-sized field");
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(ctx->ordinal));
+    const size_t n = (size_t)w * h * d;
+    int rc = ctx->field_io.reserve(n * kind_bytes(field_kind));
+    if (rc) return rc;
+    if ((rc = ctx->time_io.reserve(n * 4))) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->field_io.p, field, n * kind_bytes(field_kind), hipMemcpyHostToDevice, ctx->stream));
+    HostInst hi{ctx->field_io.p, ctx->time_io.as<float>(), w, h, seeds_xyz, n_seeds, d};
+    rc = solve_core(ctx, field_kind, &hi, 1, stats, true);
     if (rc != GHCM_OK && rc != GHCM_ERR_NOT_CONVERGED) return rc;
     HIP_TRY(hipMemcpyAsync(time_out, ctx->time_io.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

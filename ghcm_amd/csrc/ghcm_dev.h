@@ -30,14 +30,24 @@ constexpr uint32_t kMaskOpenAll = 1u << 4;    // open every cell on the next vis
 // Side numbering of the reference (src/eikonal_hcm.cpp:18, data/eikonal_common.cl:361):
 // 0: x-, 1: y-, 2: x+, 3: y+ ; opposite(s) = s ^ 2.
 
+// One instance: field and time of W x H x D cells, idx = x + W (y + H z); a 2-D instance has
+// D = 1 and tiles_z = 1.  Tile (tx, ty, tz) has the global id tile_base + tx + tiles_x (ty +
+// tiles_y tz).
 struct GhcmInst {
     const void* field;
     float*      time;
     uint32_t    w, h;
     uint32_t    tiles_x, tiles_y;
     uint32_t    tile_base;
+    uint32_t    d;
+    uint32_t    tiles_z;
     uint32_t    pad;
 };
+static_assert(sizeof(GhcmInst) == 48, "instance record layout");
+
+__host__ __device__ inline uint32_t inst_tiles(const GhcmInst& in) {
+    return in.tiles_x * in.tiles_y * in.tiles_z;
+}
 
 constexpr uint32_t kShards = 32;        // list shards
 constexpr uint32_t kCounterStride = 32; // u32 between shard counters (128 B)

@@ -19,6 +19,9 @@ struct SolveKernel {
 
 // mode: 0 systolic, 1 FIM, 2 sweep.  host: job list given by the host scheduler.
 SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host);
+// Three-dimensional tile solve (8 x 8 x 8 tiles, device scheduler) for a field kind.
+SolveKernel get_solve3d_kernel(int kind);
+constexpr uint32_t kTile3D = 8;
 // Tile solve with the wave boundary folded in (one launch per wave); sweep kernel only.
 SolveKernel get_fused_kernel(int tile_w, int tile_h, int kind, int mode);
 
@@ -50,5 +53,8 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_rgb_luminosity(const float* rgb, float* out, size_t n, hipStream_t s);
 hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
                        uint32_t tile_w, uint32_t tile_h, hipStream_t s);
+// Seeds of a 3-D solve: (x, y, z, instance) quadruples in device-readable memory.
+hipError_t launch_seed3d(const SolveArgs& a, const uint32_t* seeds_xyzi, uint32_t n_seeds,
+                         hipStream_t s);
 
 }  // namespace ghcm

@@ -201,6 +201,8 @@ __device__ __forceinline__ GhcmInst uniform_inst(const GhcmInst& m) {
     r.tiles_x = __builtin_amdgcn_readfirstlane(m.tiles_x);
     r.tiles_y = __builtin_amdgcn_readfirstlane(m.tiles_y);
     r.tile_base = __builtin_amdgcn_readfirstlane(m.tile_base);
+    r.d = __builtin_amdgcn_readfirstlane(m.d);
+    r.tiles_z = __builtin_amdgcn_readfirstlane(m.tiles_z);
     r.pad = 0u;
     return r;
 }
@@ -1092,7 +1094,7 @@ __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_ins
     for (uint32_t j = blockIdx.y; j < n_inst; j += gridDim.y) {
         const GhcmInst in = h_inst[j];
         if (blockIdx.x == 0 && threadIdx.x == 0) const_cast<GhcmInst*>(a.inst)[j] = in;
-        const uint32_t nt = in.tiles_x * in.tiles_y;
+        const uint32_t nt = inst_tiles(in);
         const uint32_t stride = gridDim.x * blockDim.x;
         for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
             const size_t g = in.tile_base + t;
@@ -1108,7 +1110,7 @@ __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_ins
             if (a.tinst) const_cast<uint32_t*>(a.tinst)[g] = j;
         }
         auto* tm = gmem(reinterpret_cast<uint32_t*>(in.time));
-        const size_t cells = (size_t)in.w * in.h;
+        const size_t cells = (size_t)in.w * in.h * in.d;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += stride)
             tm[i] = kInfBits;
     }
@@ -1251,7 +1253,7 @@ __global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t 
         const_cast<GhcmInst*>(a.inst)[slot] = in;
         const_cast<uint32_t*>(a.sgen)[slot] = gen;
     }
-    const uint32_t base = in.tile_base, count = in.tiles_x * in.tiles_y;
+    const uint32_t base = in.tile_base, count = inst_tiles(in);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const uint32_t t = base + i;
         a.qwave[t] = 0u;
@@ -1385,7 +1387,7 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 __global__ void k_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst) {
     for (uint32_t j = blockIdx.x; j < n_inst; j += gridDim.x) {
         const GhcmInst in = inst[j];
-        const uint32_t nt = in.tiles_x * in.tiles_y;
+        const uint32_t nt = inst_tiles(in);
         for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) tinst[in.tile_base + t] = j;
     }
 }
@@ -1440,6 +1442,263 @@ __global__ void k_rgb_luminosity(const float* __restrict__ rgb, float* __restric
     }
     for (size_t i = done + tid; i < n; i += stride)
         out[i] = luminosity(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Three-dimensional solve (K = 3): W x H x D grids in 8 x 8 x 8 tiles
+// ---------------------------------------------------------------------------------------
+
+// Godunov upwind update in three dimensions for |grad T| = f, h = 1 (the K-axis update of
+// src/eikonal.h:73-118 and data/eikonal_common.cl:221-285 at K = 3), on the three axis
+// minima a, b, c in any order.  Sorted m1 <= m2 <= m3, the update takes the first of
+//   m1 + f                                    if it does not exceed m2,
+//   the two-axis root (godunov_raw)           if it does not exceed m3,
+//   m1 + (u + v + sqrt(3 f^2 - u^2 - v^2 - (u - v)^2)) / 3, u = m2 - m1, v = m3 - m1,
+// the larger root of (T-m1)^2 + (T-m2)^2 + (T-m3)^2 = f^2 with the common part m1 taken out
+// of the discriminant (the reference forms it from the unshifted values).  As in two
+// dimensions, m1 is added by one final rounding, so a result never rounds below the
+// largest minimum it used.  +INF minima fall through to the lower-axis forms; f = +INF
+// yields +INF.
+__device__ __forceinline__ float godunov3_raw(float a, float b, float c, float f) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    const float m1 = fminf(lo, c);
+    const float x = fmaxf(lo, c);
+    const float m2 = fminf(x, hi), m3 = fmaxf(x, hi);
+    const float t1 = m1 + f;
+    if (!(t1 > m2)) return t1;
+    const float t2 = godunov_raw(m1, m2, m1, f);
+    if (!(t2 > m3)) return t2;
+    const float u = m2 - m1, v = m3 - m1, w = u - v;
+    const float disc = fmaxf(3.0f * f * f - (u * u + v * v + w * w), 0.0f);
+    return m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
+}
+
+constexpr int kT3 = 8;                 // 3-D tile edge
+constexpr int kT3Threads = 256;        // two cells per thread
+constexpr int kT3Box = kT3 + 2;        // staged box edge (tile plus a one-cell halo)
+constexpr uint32_t kT3HardPasses = 4 * kT3 * kT3 * kT3 + 64;
+
+// Side numbering in three dimensions: 0 x-, 1 y-, 2 x+, 3 y+ (as in two), 4 z-, 5 z+.
+__device__ __forceinline__ void side_step3(int s, int& dx, int& dy, int& dz) {
+    dx = s == 0 ? -1 : s == 2 ? 1 : 0;
+    dy = s == 1 ? -1 : s == 3 ? 1 : 0;
+    dz = s == 4 ? -1 : s == 5 ? 1 : 0;
+}
+
+// One visit of 3-D tile g: stage the tile and its halo box in LDS, relax every cell in
+// Jacobi passes (each thread owns two cells and updates them in place) until no cell
+// improves by >= eps or the pass cap, predict the six neighbour faces, activate the
+// neighbours whose face would improve, commit the improved cells.
+template <int KIND>
+__device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
+                             uint32_t p, float* t_s, float* c_s, uint32_t* s_u32, float* s_f32,
+                             GhcmInst in_single) {
+    constexpr int B = kT3, PB = kT3Box, PB2 = PB * PB, PP = PB2 * PB, P = B * B * B;
+    constexpr int CPT = P / kT3Threads;
+    const int tid = threadIdx.x;
+    GhcmInst in = in_single;
+    if (a.n_inst != 1) in = uniform_inst(a.inst[a.tinst[g]]);
+    const uint32_t lt = g - in.tile_base;
+    const int tx = (int)(lt % in.tiles_x);
+    const int ty = (int)((lt / in.tiles_x) % in.tiles_y);
+    const int tz = (int)(lt / (in.tiles_x * in.tiles_y));
+    const int W = (int)in.w, H = (int)in.h, D = (int)in.d;
+    const int ox = tx * B, oy = ty * B, oz = tz * B;
+
+    // Stage the PB^3 box (origin (ox-1, oy-1, oz-1)); out-of-grid cells read +INF.
+    {
+        constexpr int NL = (PP + kT3Threads - 1) / kT3Threads;
+        float tv[NL], cv[NL];
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int i = tid + kT3Threads * k;
+            const int bx = i % PB, by = (i / PB) % PB, bz = i / PB2;
+            const int gx = ox + bx - 1, gy = oy + by - 1, gz = oz + bz - 1;
+            tv[k] = __builtin_inff();
+            cv[k] = __builtin_inff();
+            if (i < PP && gx >= 0 && gy >= 0 && gz >= 0 && gx < W && gy < H && gz < D) {
+                const size_t idx = ((size_t)gz * (size_t)H + (size_t)gy) * (size_t)W + (size_t)gx;
+                tv[k] = gmem(in.time)[idx];
+                cv[k] = load_cost_at<KIND>(static_cast<const char*>(in.field) + idx * field_bytes<KIND>());
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            const int i = tid + kT3Threads * k;
+            if (i < PP) {
+                t_s[i] = tv[k];
+                c_s[i] = cv[k];
+            }
+        }
+    }
+    __syncthreads();
+    int q[CPT];
+    float told[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = tid + kT3Threads * k;
+        const int x = i % B, y = (i / B) % B, z = i / (B * B);
+        q[k] = ((z + 1) * PB + (y + 1)) * PB + (x + 1);
+        told[k] = t_s[q[k]];
+    }
+
+    // In-tile passes.  Out-of-grid cells hold +INF time and cost and stay +INF.
+    const float eps = a.eps;
+    const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kT3HardPasses) ? kT3HardPasses : a.iter_cap;
+    uint32_t passes = 0;
+    bool unfinished = false;
+    for (;;) {
+        bool changed = false;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int c = q[k];
+            const float t0 = t_s[c];
+            const float ax = fminf(t_s[c - 1], t_s[c + 1]);
+            const float ay = fminf(t_s[c - PB], t_s[c + PB]);
+            const float az = fminf(t_s[c - PB2], t_s[c + PB2]);
+            const float tn = godunov3_raw(ax, ay, az, c_s[c]);
+            if (tn < t0) {
+                t_s[c] = tn;
+                if (t0 - tn >= eps) changed = true;
+            }
+        }
+        ++passes;
+        const bool more = __syncthreads_or(changed) != 0;
+        if (!more) break;
+        if (passes >= cap) { unfinished = true; break; }
+    }
+
+    // Face predictions: would the neighbour's cell across face s improve by >= eps?  Its
+    // neighbour further out is not staged and counts as +INF.
+    if (tid < 6) {
+        s_u32[tid] = 0u;
+        s_f32[tid] = __builtin_inff();
+    }
+    __syncthreads();
+    for (int j = tid; j < 6 * B * B; j += kT3Threads) {
+        const int s = j / (B * B), f = j % (B * B);
+        const int u = f % B, v = f / B;
+        int dx, dy, dz;
+        side_step3(s, dx, dy, dz);
+        const int ntx = tx + dx, nty = ty + dy, ntz = tz + dz;
+        if (ntx < 0 || nty < 0 || ntz < 0 || ntx >= (int)in.tiles_x || nty >= (int)in.tiles_y ||
+            ntz >= (int)in.tiles_z)
+            continue;
+        // halo cell (bx, by, bz) in box coordinates
+        int bx, by, bz;
+        if (s == 0 || s == 2)      { bx = s == 0 ? 0 : B + 1; by = u + 1; bz = v + 1; }
+        else if (s == 1 || s == 3) { by = s == 1 ? 0 : B + 1; bx = u + 1; bz = v + 1; }
+        else                       { bz = s == 4 ? 0 : B + 1; bx = u + 1; by = v + 1; }
+        const int gx = ox + bx - 1, gy = oy + by - 1, gz = oz + bz - 1;
+        if (gx >= W || gy >= H || gz >= D) continue;
+        const int c = (bz * PB + by) * PB + bx;
+        float ax, ay, az;
+        if (s == 0 || s == 2) ax = t_s[c + (s == 0 ? 1 : -1)];
+        else ax = fminf(t_s[c - 1], t_s[c + 1]);
+        if (s == 1 || s == 3) ay = t_s[c + (s == 1 ? PB : -PB)];
+        else ay = fminf(t_s[c - PB], t_s[c + PB]);
+        if (s == 4 || s == 5) az = t_s[c + (s == 4 ? PB2 : -PB2)];
+        else az = fminf(t_s[c - PB2], t_s[c + PB2]);
+        const float t0 = t_s[c];
+        const float tn = godunov3_raw(ax, ay, az, c_s[c]);
+        if (tn < t0 && t0 - tn >= eps) {
+            atomicOr(&s_u32[s], 1u);
+            atomicMin(reinterpret_cast<uint32_t*>(s_f32) + s, __float_as_uint(tn));
+        }
+    }
+    __syncthreads();
+    if (tid < 6 && s_u32[tid]) {
+        int dx, dy, dz;
+        side_step3(tid, dx, dy, dz);
+        const uint32_t nb = in.tile_base + (uint32_t)(tx + dx) +
+                            in.tiles_x * ((uint32_t)(ty + dy) + in.tiles_y * (uint32_t)(tz + dz));
+        activate(a, nb, wave, cur, p, kMaskOpenAll, s_f32[tid]);
+    }
+
+    // Commit improved cells (one writer per tile per wave), tile minimum.
+    float tmin = __builtin_inff();
+    uint32_t improved = 0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = tid + kT3Threads * k;
+        const int x = i % B, y = (i / B) % B, z = i / (B * B);
+        const float tn = t_s[q[k]];
+        if (ox + x < W && oy + y < H && oz + z < D) {
+            tmin = fminf(tmin, tn);
+            if (tn < told[k]) {
+                const size_t idx = ((size_t)(oz + z) * (size_t)H + (size_t)(oy + y)) * (size_t)W + (size_t)(ox + x);
+                gmem(in.time)[idx] = tn;
+                ++improved;
+            }
+        }
+    }
+    tmin = wave_min(tmin);
+    improved = wave_sum(improved);
+    const int w = tid >> 6, L = tid & 63;
+    if (L == 0) {
+        s_f32[8 + w] = tmin;
+        s_u32[8 + w] = improved;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const float tm = fminf(fminf(s_f32[8], s_f32[9]), fminf(s_f32[10], s_f32[11]));
+        a.tmin[g] = tm;
+        stat_add(&a.tstat[g], 1u);
+        stat_add(&a.tstat[(size_t)a.tiles + g], passes);
+        stat_add(&a.tstat[2 * (size_t)a.tiles + g], s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11]);
+        if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tm);
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kT3Threads) void k_solve3d(SolveArgs a) {
+    __shared__ float t_s[kT3Box * kT3Box * kT3Box];
+    __shared__ float c_s[kT3Box * kT3Box * kT3Box];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[16];
+    __shared__ uint32_t s_prefix[kShards + 1];
+    const int tid = threadIdx.x;
+    const GhcmInst in0 = single_inst(a);
+    const GhcmCtrl* ctrl = a.ctrl;
+    publish_progress(a, ctrl);
+    const uint32_t done = ctrl->done;
+    const uint32_t nj = ctrl->n_jobs;
+    const uint32_t wave = ctrl->wave;
+    const uint32_t cur = ctrl->cur;
+    const uint32_t from_list = ctrl->jobs_from_list;
+    const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
+    if (done || blockIdx.x >= nj) return;
+    const uint32_t p = wave & 1u;
+    if (tid <= (int)kShards) s_prefix[tid] = pv;
+    __syncthreads();
+    for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
+        const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+        if (tid == 0) claim_tile(a, g, p);
+        solve3d_tile<KIND>(a, g, wave, cur, p, t_s, c_s, s_u32, s_f32, in0);
+        __syncthreads();
+    }
+}
+
+// Seeds of 3-D solves: (x, y, z, instance) quadruples; T = 0 at the seed, and the tiles of
+// the seed and of its six neighbours are queued for the first wave with every cell open.
+__global__ void k_seed3d(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_seeds) return;
+    const uint32_t x = seeds[4 * s], y = seeds[4 * s + 1], z = seeds[4 * s + 2], ii = seeds[4 * s + 3];
+    const GhcmInst in = a.inst[ii];
+    gmem(in.time)[((size_t)z * in.h + y) * in.w + x] = 0.0f;
+    for (int d = 0; d < 7; ++d) {
+        int dx = 0, dy = 0, dz = 0;
+        if (d > 0) side_step3(d - 1, dx, dy, dz);
+        const int px = (int)x + dx, py = (int)y + dy, pz = (int)z + dz;
+        if (px < 0 || py < 0 || pz < 0 || px >= (int)in.w || py >= (int)in.h || pz >= (int)in.d) continue;
+        const uint32_t t = in.tile_base + (uint32_t)(px / kT3) +
+                           in.tiles_x * ((uint32_t)(py / kT3) + in.tiles_y * (uint32_t)(pz / kT3));
+        atomicMin(&a.key[t], 0u);
+        atomicOr(&a.mask[t], kMaskOpenAll);
+        const uint32_t old = atomicMax(&a.qwave[t], 1u);
+        if (old < 1u) list_append(a, 0u, t);
+    }
 }
 
 // ---- launcher table ----
@@ -1650,7 +1909,7 @@ hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds, uint32_t
 
 hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
                               uint32_t gen, hipStream_t s) {
-    const uint32_t count = in.tiles_x * in.tiles_y;
+    const uint32_t count = inst_tiles(in);
     unsigned blocks = (count + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
@@ -1696,6 +1955,40 @@ hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) 
     if (blocks < 1) blocks = 1;
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, bytes);
+    return hipGetLastError();
+}
+
+template <int KIND>
+static hipError_t launch_solve3d_t(const SolveArgs& a, uint32_t grid, hipStream_t s, hipEvent_t ev0,
+                                   hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve3d<KIND>), dim3(grid), dim3(kT3Threads), 0, s, ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL((k_solve3d<KIND>), dim3(grid), dim3(kT3Threads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int KIND>
+static int occ_solve3d_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve3d<KIND>, kT3Threads, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+SolveKernel get_solve3d_kernel(int kind) {
+    switch (kind) {
+        case 0: return {&launch_solve3d_t<0>, &occ_solve3d_t<0>};
+        case 1: return {&launch_solve3d_t<1>, &occ_solve3d_t<1>};
+        case 2: return {&launch_solve3d_t<2>, &occ_solve3d_t<2>};
+        case 3: return {&launch_solve3d_t<3>, &occ_solve3d_t<3>};
+        default: return {nullptr, nullptr};
+    }
+}
+
+hipError_t launch_seed3d(const SolveArgs& a, const uint32_t* seeds, uint32_t n_seeds, hipStream_t s) {
+    if (n_seeds == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_seed3d, dim3((n_seeds + 63) / 64), dim3(64), 0, s, a, seeds, n_seeds);
     return hipGetLastError();
 }
 

@@ -463,6 +463,8 @@ int ghcm_server::admit(std::vector<Request*>& batch, uint32_t enqueued) {
         g.tiles_x = r->tiles_x;
         g.tiles_y = r->tiles_y;
         g.tile_base = (uint32_t)r->tile_base;
+        g.d = 1;
+        g.tiles_z = 1;
         r->gen = next_gen++;
         if (next_gen == 0) next_gen = 1;
         HIP_TRY(launch_admit_tiles(a, g, r->slot, r->gen, stream));

@@ -8,6 +8,10 @@ Batch            `n` instances, W and H drawn independently from {64, 128, ..., 
                  (sigma 32, as the 4096^2 field) chosen per instance; one seed for all.
 Serving          square sizes log-uniform in [128, 4096] (rounded to 64), 1-4 sources,
                  ring or smooth (sigma 32); Poisson arrival times at a given rate.
+Volume (256^3)   3-D smooth log-normal speed in [0.1, 10]: iid N(0, 1) from numpy's PCG64
+                 with the given seed, a Gaussian filter of sigma 8 (scipy.ndimage, reflect
+                 boundary, truncated at 4 sigma), mapped log-linearly like the 2-D field;
+                 4 point sources.
 
 The per-instance streams use splitmix64 (the library's generator) through
 ``gen_points``-style draws so the same seed gives the same batch on any host.
@@ -44,6 +48,25 @@ class Instance:
 def plain_4096():
     speed = gen_smooth_speed(4096, 4096, PLAIN_SEED, 32.0, 0.1, 10.0)
     return Instance(4096, 4096, "smooth", speed, gen_points(4096, 4096, PLAIN_POINT_SEED, 4))
+
+
+def smooth_speed_3d(n: int, seed: int, sigma: float = 8.0, lo: float = 0.1,
+                    hi: float = 10.0) -> np.ndarray:
+    """(n, n, n) smooth log-normal speed field in [lo, hi] (float32)."""
+    from scipy.ndimage import gaussian_filter
+    z = np.random.default_rng(seed).standard_normal((n, n, n), dtype=np.float32)
+    z = gaussian_filter(z, sigma, mode="reflect", truncate=4.0)
+    zmin, zmax = float(z.min()), float(z.max())
+    e = np.log10(lo) + (np.log10(hi) - np.log10(lo)) * (z - zmin) / (zmax - zmin)
+    return np.power(np.float32(10.0), e.astype(np.float32)).astype(np.float32)
+
+
+def volume_256(seed: int = 7):
+    """The 3-D workload: 256^3 smooth speed, 4 sources; returns (speed, seeds (x, y, z))."""
+    n = 256
+    speed = smooth_speed_3d(n, seed)
+    pts = np.random.default_rng(seed + 1).integers(0, n, (4, 3)).astype(np.uint32)
+    return speed, pts
 
 
 def rings_256(seed: int = 1):

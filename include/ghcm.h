@@ -166,6 +166,22 @@ int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind,
                       uint32_t w, uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
                       float* d_time, ghcm_stats* stats);
 
+/* ---- three-dimensional solve (the K = 3 generalisation of the update,
+ * eikonal_solve<K>, src/eikonal.h:73-118 and data/eikonal_common.cl:221-285; the
+ * reference declares K <= 3 but solves only 2-D grids) ----------------------------
+ * `field` and the time are W*H*D cells, idx = x + W*(y + H*z); `seeds_xyz` holds
+ * n_seeds (x, y, z) triples (host memory).  Tiles are 8x8x8 and the device scheduler
+ * runs the wave loop: n_active, min_change, inner_iter_cap, waves_per_poll, max_waves
+ * and launch_mode 2 (graph replay) apply, the tile shape and kernel fields do not, and
+ * the host scheduler returns GHCM_ERR_UNSUPPORTED.  Host-buffer and device-buffer forms
+ * as for the 2-D solve. */
+int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
+                 uint32_t d, const uint32_t* seeds_xyz, uint32_t n_seeds, float* time_out,
+                 ghcm_stats* stats);
+int ghcm_solve3d_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
+                        uint32_t h, uint32_t d, const uint32_t* seeds_xyz, uint32_t n_seeds,
+                        float* d_time, ghcm_stats* stats);
+
 /* ---- device memory on the context's device (plumbing for ghcm_solve_device and
  * ghcm_solve_batch, so hosts need no other GPU runtime in-process) ---------------- */
 int ghcm_dev_alloc(ghcm_ctx* ctx, size_t bytes, void** d_out);

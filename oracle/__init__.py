@@ -50,6 +50,16 @@ def lib() -> ctypes.CDLL:
         L.orc_fim_f32.argtypes = [f32, vp, u32, u32, vp, u32, vp]
         L.orc_sweep_f64.restype = ctypes.c_int
         L.orc_sweep_f64.argtypes = [vp, u32, u32, vp, u32, vp, ctypes.c_int]
+        L.orc_godunov_k_f32.restype = f32
+        L.orc_godunov_k_f32.argtypes = [vp, ctypes.c_int, f32, f32]
+        L.orc_godunov_k_f64.restype = f64
+        L.orc_godunov_k_f64.argtypes = [vp, ctypes.c_int, f64, f64]
+        L.orc_fmm3d_f32.restype = ctypes.c_int
+        L.orc_fmm3d_f32.argtypes = [vp, u32, u32, u32, vp, u32, vp]
+        L.orc_fmm3d_f64.restype = ctypes.c_int
+        L.orc_fmm3d_f64.argtypes = [vp, u32, u32, u32, vp, u32, vp]
+        L.orc_sweep3d_f64.restype = ctypes.c_int
+        L.orc_sweep3d_f64.argtypes = [vp, u32, u32, u32, vp, u32, vp, ctypes.c_int]
         L.orc_error_metrics.restype = None
         L.orc_error_metrics.argtypes = [vp, vp, ctypes.c_size_t, vp]
         _lib = L
@@ -126,6 +136,56 @@ def sweep_f64(cost: np.ndarray, seeds, max_rounds: int = 10000):
     r = lib().orc_sweep_f64(_ptr(c), w, h, _ptr(s), s.shape[0], _ptr(out), max_rounds)
     if r < 0:
         raise ValueError("orc_sweep_f64 failed")
+    return out, int(r)
+
+
+def godunov_k_f32(dims, cost, best_prev=np.inf) -> float:
+    """eikonal_solve<K> in FP32 (K = len(dims), 1..3), larger root."""
+    d = np.ascontiguousarray(dims, dtype=np.float32)
+    return lib().orc_godunov_k_f32(_ptr(d), d.size, cost, best_prev)
+
+
+def godunov_k_f64(dims, cost, best_prev=np.inf) -> float:
+    """eikonal_solve<K> in FP64 (K = len(dims), 1..3), larger root."""
+    d = np.ascontiguousarray(dims, dtype=np.float64)
+    return lib().orc_godunov_k_f64(_ptr(d), d.size, cost, best_prev)
+
+
+def _seeds3(seeds) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(seeds, dtype=np.uint32).reshape(-1, 3))
+
+
+def fmm3d_f32(cost: np.ndarray, seeds) -> np.ndarray:
+    """Three-dimensional FMM in FP32; cost is (D, H, W), seeds (x, y, z)."""
+    c = np.ascontiguousarray(cost, dtype=np.float32)
+    d, h, w = c.shape
+    s = _seeds3(seeds)
+    out = np.empty_like(c)
+    if lib().orc_fmm3d_f32(_ptr(c), w, h, d, _ptr(s), s.shape[0], _ptr(out)) != 0:
+        raise ValueError("orc_fmm3d_f32 failed")
+    return out
+
+
+def fmm3d_f64(cost: np.ndarray, seeds) -> np.ndarray:
+    """Three-dimensional FMM in FP64 -- the accuracy oracle in 3-D."""
+    c = np.ascontiguousarray(cost, dtype=np.float64)
+    d, h, w = c.shape
+    s = _seeds3(seeds)
+    out = np.empty_like(c)
+    if lib().orc_fmm3d_f64(_ptr(c), w, h, d, _ptr(s), s.shape[0], _ptr(out)) != 0:
+        raise ValueError("orc_fmm3d_f64 failed")
+    return out
+
+
+def sweep3d_f64(cost: np.ndarray, seeds, max_rounds: int = 10000):
+    """Three-dimensional Gauss-Seidel fixed point in FP64; returns (times, rounds)."""
+    c = np.ascontiguousarray(cost, dtype=np.float64)
+    d, h, w = c.shape
+    s = _seeds3(seeds)
+    out = np.empty_like(c)
+    r = lib().orc_sweep3d_f64(_ptr(c), w, h, d, _ptr(s), s.shape[0], _ptr(out), max_rounds)
+    if r < 0:
+        raise ValueError("orc_sweep3d_f64 failed")
     return out, int(r)
 
 

@@ -334,6 +334,154 @@ int orc_sweep_f64(const double* cost, uint32_t w, uint32_t h,
 }
 
 /* ------------------------------------------------------------------------- */
+/* K-axis update and the three-dimensional solvers                             */
+/* ------------------------------------------------------------------------- */
+
+/* eikonal_solve<K> (src/eikonal.h:73-118): axis minima sorted ascending (the bubble sort
+ * of eikonal_axis_sort, data/eikonal_common.cl:215-219), then for i = 0..K-1: stop once
+ * the best time does not exceed axis value i; stop at an INF axis; otherwise accumulate
+ * the quadratic sum_{j<=i} (T - a_j)^2 = cost^2 (b += -2 a_i, c += a_i^2 with c starting
+ * at -cost^2) and take its root, a_0 + cost at i = 0.  The reference's release build
+ * takes the smaller root (-b - sqrt(quad)) / 2a at :101, which lies below a_i; the
+ * scheme's root is the larger one, taken here. */
+#define GODUNOV_K_BODY(T, SQRT)                                                            \
+    T v[3];                                                                                \
+    if (K < 1 || K > 3) return best_prev;                                                   \
+    for (int i = 0; i < K; ++i) v[i] = dims[i];                                             \
+    for (int i = 0; i < K; ++i)                                                             \
+        for (int j = 0; j + 1 < K - i; ++j)                                                 \
+            if (v[j + 1] < v[j]) { T x = v[j]; v[j] = v[j + 1]; v[j + 1] = x; }             \
+    T b = 0, c = -(cost * cost), best = best_prev;                                          \
+    for (int i = 0; i < K; ++i) {                                                           \
+        T ta = v[i];                                                                        \
+        if (best <= ta) break;                                                              \
+        if (isinf(ta)) return best;                                                         \
+        b += -2 * ta;                                                                       \
+        c += ta * ta;                                                                       \
+        if (i == 0) {                                                                       \
+            T t = ta + cost;                                                                \
+            best = t < best ? t : best;                                                     \
+        } else {                                                                            \
+            T a = (T)(i + 1);                                                               \
+            T quad = b * b - 4 * a * c;                                                     \
+            if (quad < 0) quad = 0;                                                         \
+            T t = (-b + SQRT(quad)) / (2 * a);                                              \
+            best = t < best ? t : best;                                                     \
+        }                                                                                   \
+    }                                                                                       \
+    return best;
+
+float orc_godunov_k_f32(const float* dims, int K, float cost, float best_prev) {
+    GODUNOV_K_BODY(float, sqrtf)
+}
+
+double orc_godunov_k_f64(const double* dims, int K, double cost, double best_prev) {
+    GODUNOV_K_BODY(double, sqrt)
+}
+
+/* Neighbour visit order: -x, +x, -y, +y (src/eikonal_fmm.cpp:61-62), then -z, +z. */
+static const int k3_off[6][3] = { {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1} };
+
+/* Axis minima of cell (x, y, z) of a W x H x D field, out-of-grid reads +INF. */
+#define AXIS_MINS_3D(T, t, x, y, z, out)                                                    \
+    do {                                                                                    \
+        size_t ci_ = (size_t)(x) + (size_t)w * ((size_t)(y) + (size_t)h * (size_t)(z));     \
+        T l_ = (x) > 0 ? t[ci_ - 1] : (T)INFINITY;                                           \
+        T r_ = (x) + 1 < (int)w ? t[ci_ + 1] : (T)INFINITY;                                  \
+        T u_ = (y) > 0 ? t[ci_ - w] : (T)INFINITY;                                           \
+        T d_ = (y) + 1 < (int)h ? t[ci_ + w] : (T)INFINITY;                                  \
+        T f_ = (z) > 0 ? t[ci_ - (size_t)w * h] : (T)INFINITY;                               \
+        T b_ = (z) + 1 < (int)dd ? t[ci_ + (size_t)w * h] : (T)INFINITY;                     \
+        out[0] = l_ < r_ ? l_ : r_;                                                         \
+        out[1] = u_ < d_ ? u_ : d_;                                                         \
+        out[2] = f_ < b_ ? f_ : b_;                                                         \
+    } while (0)
+
+/* FMM in three dimensions: the sequential FMM of src/eikonal_fmm.cpp:12-71 with six
+ * neighbours and the K = 3 update from accepted values. */
+#define FMM3D_BODY(KEY_T, HEAP, GODUNOV_K)                                                   \
+    if (!cost || !time_out || w == 0 || h == 0 || dd == 0 || (n_seeds && !seeds)) return -1; \
+    size_t N = (size_t)w * h * dd;                                                           \
+    for (size_t i = 0; i < N; ++i) time_out[i] = (KEY_T)INFINITY;                            \
+    HEAP hp = { 0, 0, 0 };                                                                   \
+    for (uint32_t s = 0; s < n_seeds; ++s) {                                                 \
+        uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], z = seeds[3 * s + 2];               \
+        if (x >= w || y >= h || z >= dd) { free(hp.a); return -1; }                          \
+        if (HEAP##_push(&hp, (KEY_T)0, (uint32_t)(x + (size_t)w * (y + (size_t)h * z)))) {   \
+            free(hp.a); return -1;                                                           \
+        }                                                                                    \
+    }                                                                                        \
+    while (hp.n) {                                                                           \
+        HEAP##_ent cur = HEAP##_pop(&hp);                                                    \
+        if (time_out[cur.idx] <= cur.key) continue;                                          \
+        time_out[cur.idx] = cur.key;                                                         \
+        int cx = (int)(cur.idx % w), cy = (int)((cur.idx / w) % h), cz = (int)(cur.idx / ((size_t)w * h)); \
+        for (int k = 0; k < 6; ++k) {                                                        \
+            int px = cx + k3_off[k][0], py = cy + k3_off[k][1], pz = cz + k3_off[k][2];      \
+            if (px < 0 || py < 0 || pz < 0 || px >= (int)w || py >= (int)h || pz >= (int)dd) continue; \
+            size_t pi = (size_t)px + (size_t)w * ((size_t)py + (size_t)h * (size_t)pz);      \
+            if (!(cost[pi] < (KEY_T)INFINITY) || time_out[pi] != (KEY_T)INFINITY) continue;  \
+            KEY_T m[3];                                                                      \
+            AXIS_MINS_3D(KEY_T, time_out, px, py, pz, m);                                    \
+            KEY_T t = GODUNOV_K(m, 3, cost[pi], (KEY_T)INFINITY);                            \
+            if (HEAP##_push(&hp, t, (uint32_t)pi)) { free(hp.a); return -1; }                \
+        }                                                                                    \
+    }                                                                                        \
+    free(hp.a);                                                                              \
+    return 0;
+
+int orc_fmm3d_f32(const float* cost, uint32_t w, uint32_t h, uint32_t dd,
+                  const uint32_t* seeds, uint32_t n_seeds, float* time_out) {
+    FMM3D_BODY(float, heap32, orc_godunov_k_f32)
+}
+
+int orc_fmm3d_f64(const double* cost, uint32_t w, uint32_t h, uint32_t dd,
+                  const uint32_t* seeds, uint32_t n_seeds, double* time_out) {
+    FMM3D_BODY(double, heap64, orc_godunov_k_f64)
+}
+
+/* Fast-sweeping Gauss-Seidel in three dimensions (eight orderings) to a fixed point, FP64. */
+int orc_sweep3d_f64(const double* cost, uint32_t w, uint32_t h, uint32_t dd,
+                    const uint32_t* seeds, uint32_t n_seeds, double* t, int max_rounds) {
+    if (!cost || !t || w == 0 || h == 0 || dd == 0) return -1;
+    size_t N = (size_t)w * h * dd;
+    uint8_t* fixed = (uint8_t*)calloc(N, 1);
+    if (!fixed) return -1;
+    for (size_t i = 0; i < N; ++i) t[i] = INFINITY;
+    for (uint32_t s = 0; s < n_seeds; ++s) {
+        uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], z = seeds[3 * s + 2];
+        if (x >= w || y >= h || z >= dd) { free(fixed); return -1; }
+        size_t i = x + (size_t)w * (y + (size_t)h * z);
+        t[i] = 0.0; fixed[i] = 1;
+    }
+    int rounds = 0;
+    for (; rounds < max_rounds; ++rounds) {
+        int changed = 0;
+        for (int dir = 0; dir < 8; ++dir) {
+            int xs = (dir & 1) ? -1 : 1, ys = (dir & 2) ? -1 : 1, zs = (dir & 4) ? -1 : 1;
+            for (uint32_t kk = 0; kk < dd; ++kk) {
+                int z = zs > 0 ? (int)kk : (int)(dd - 1 - kk);
+                for (uint32_t jj = 0; jj < h; ++jj) {
+                    int y = ys > 0 ? (int)jj : (int)(h - 1 - jj);
+                    for (uint32_t ii = 0; ii < w; ++ii) {
+                        int x = xs > 0 ? (int)ii : (int)(w - 1 - ii);
+                        size_t i = (size_t)x + (size_t)w * ((size_t)y + (size_t)h * (size_t)z);
+                        if (fixed[i]) continue;
+                        double m[3];
+                        AXIS_MINS_3D(double, t, x, y, z, m);
+                        double nv = orc_godunov_k_f64(m, 3, cost[i], t[i]);
+                        if (nv < t[i]) { t[i] = nv; changed = 1; }
+                    }
+                }
+            }
+        }
+        if (!changed) { ++rounds; break; }
+    }
+    free(fixed);
+    return rounds;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Error metrics                                                                */
 /* ------------------------------------------------------------------------- */
 

@@ -25,6 +25,16 @@
  *       pinned against orc_fmm_f64 and the closed forms above (each pins the
  *       other: they share no code beyond the Godunov update, which is pinned
  *       separately).
+ *   orc_godunov_k_f32 / orc_godunov_k_f64
+ *       closed forms for K = 1, 2, 3 (a + c, the 2-D KATs, a + c/sqrt3 for three
+ *       equal minima), agreement with the 2-D forms when the third axis is INF, the
+ *       residual sum (T - a_i)_+^2 = c^2, monotonicity (tests/test_oracle_kd.py).
+ *   orc_fmm3d_f32 / orc_fmm3d_f64
+ *       1-D strips (prefix sums), D = 1 slabs equal to the 2-D FMM, constant-cost
+ *       closed forms at (i,0,0), (1,1,0), (1,1,1), Euclidean <= T <= Manhattan,
+ *       obstacles, and the 3-D sweeping fixed point (tests/test_oracle_kd.py).
+ *   orc_sweep3d_f64
+ *       pinned against orc_fmm3d_f64 and the closed forms above.
  *   orc_error_metrics
  *       hand-computed small arrays.
  */
@@ -54,6 +64,26 @@ float orc_godunov_safe(float ax, float ay, float cost, float t_curr);
 
 /* Same update in FP64 (no quirk; always min'ed). */
 double orc_godunov_f64(double ax, double ay, double cost, double t_curr);
+
+/* Generic K-axis update, eikonal_solve<K> (src/eikonal.h:73-118), K = 1..3, in FP32
+ * and FP64: the K axis minima are sorted inside; the result is min'ed with best_prev.
+ * The larger root of each quadratic is taken (the reference's release build takes the
+ * smaller one, src/eikonal.h:101). */
+float orc_godunov_k_f32(const float* dims, int K, float cost, float best_prev);
+double orc_godunov_k_f64(const double* dims, int K, double cost, double best_prev);
+
+/* Three-dimensional FMM: W x H x D, idx = x + W*(y + H*z), seeds as (x,y,z) triples,
+ * six neighbours, the K = 3 update from accepted values (FP32 with orc_godunov_k_f32,
+ * FP64 with orc_godunov_k_f64).  Returns 0, or -1 on bad arguments. */
+int orc_fmm3d_f32(const float* cost, uint32_t w, uint32_t h, uint32_t d,
+                  const uint32_t* seeds, uint32_t n_seeds, float* time_out);
+int orc_fmm3d_f64(const double* cost, uint32_t w, uint32_t h, uint32_t d,
+                  const uint32_t* seeds, uint32_t n_seeds, double* time_out);
+
+/* Three-dimensional fast-sweeping Gauss-Seidel (eight orderings) to a fixed point, FP64.
+ * Returns the rounds used, or -1 on bad arguments. */
+int orc_sweep3d_f64(const double* cost, uint32_t w, uint32_t h, uint32_t d,
+                    const uint32_t* seeds, uint32_t n_seeds, double* t, int max_rounds);
 
 /* Sequential FMM (src/eikonal_fmm.cpp:12-71): lazy-deletion binary min-heap,
  * neighbours solved from accepted values only.  cost[W*H], seeds[2*n] as

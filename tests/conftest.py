@@ -41,6 +41,40 @@ def fixed_point_gap(t: np.ndarray, cost: np.ndarray, seeds) -> np.ndarray:
     return gap
 
 
+def godunov3_f64(a, b, c, cost):
+    """The K = 3 discrete update in FP64 on arrays: the larger root of
+    sum_i max(T - m_i, 0)^2 = cost^2 over the sorted axis minima m_0 <= m_1 <= m_2, i.e. the
+    first of m_0 + cost (if <= m_1), the two-axis root (if <= m_2), the three-axis root."""
+    m = np.sort(np.stack([np.asarray(a, np.float64), np.asarray(b, np.float64),
+                          np.asarray(c, np.float64)]), axis=0)
+    C = np.asarray(cost, dtype=np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        t1 = m[0] + C
+        d = m[1] - m[0]
+        two = (m[0] + m[1] + np.sqrt(np.maximum(2.0 * C * C - d * d, 0.0))) / 2.0
+        s = m[0] + m[1] + m[2]
+        q = s * s - 3.0 * (m[0] ** 2 + m[1] ** 2 + m[2] ** 2 - C * C)
+        three = (s + np.sqrt(np.maximum(q, 0.0))) / 3.0
+        return np.where(t1 <= m[1], t1, np.where(two <= m[2], two, three))
+
+
+def fixed_point_gap3d(t: np.ndarray, cost: np.ndarray, seeds) -> np.ndarray:
+    """|T - G3(a, b, c, cost)| at every non-seed finite cell of a (D, H, W) field, with the
+    axis minima of the six neighbours (out of grid = +INF) and G3 as in godunov3_f64."""
+    T = np.asarray(t, dtype=np.float64)
+    P = np.pad(T, 1, constant_values=np.inf)
+    a = np.minimum(P[1:-1, 1:-1, :-2], P[1:-1, 1:-1, 2:])
+    b = np.minimum(P[1:-1, :-2, 1:-1], P[1:-1, 2:, 1:-1])
+    c = np.minimum(P[:-2, 1:-1, 1:-1], P[2:, 1:-1, 1:-1])
+    G = godunov3_f64(a, b, c, cost)
+    with np.errstate(invalid="ignore"):
+        gap = np.abs(T - G)
+    gap[~np.isfinite(T) | ~np.isfinite(np.asarray(cost, dtype=np.float64))] = 0.0
+    for x, y, z in np.asarray(seeds).reshape(-1, 3):
+        gap[z, y, x] = 0.0
+    return gap
+
+
 def ulp32(t: np.ndarray) -> np.ndarray:
     """Spacing of FP32 numbers at |t| (finite entries)."""
     a = np.abs(np.asarray(t, dtype=np.float32))

@@ -1,0 +1,116 @@
+"""Three-dimensional solves (the K = 3 generalisation of src/eikonal.h:73-118) on the GPU,
+checked against the oracle's 3-D FMM in FP64 -- needs an MI355X."""
+import numpy as np
+import pytest
+
+import ghcm_amd as g
+import oracle
+from conftest import fixed_point_gap3d, ulp32
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4      # max |dT| <= TOL * max T, as for the plain 2-D form
+
+
+def _check(t, ref, cost, seeds, eps):
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(t), fin)
+    for x, y, z in np.asarray(seeds).reshape(-1, 3):
+        assert t[z, y, x] == 0.0
+    err = np.max(np.abs(t[fin].astype(np.float64) - ref[fin])) if fin.any() else 0.0
+    assert err <= TOL * ref[fin].max(), err
+    gap = fixed_point_gap3d(t, cost, seeds)
+    assert np.all(gap <= 4 * ulp32(t) + 2 * eps + 1e-6)
+    return err
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (17, 23, 29), (40, 9, 33)])
+def test_3d_parity_random_with_walls(shape, rng):
+    d, h, w = shape
+    print(f"case 3d random {shape}", flush=True)
+    cost = rng.uniform(0.1, 10.0, shape).astype(np.float32)
+    if d > 8:
+        cost[d // 2, 1:h - 1, 2:w - 2] = np.inf          # a wall with a rim to go around
+    seeds = [(0, 0, 0), (w - 1, h - 1, d - 1), (w // 2, 0, d // 3)]
+    ref = oracle.fmm3d_f64(cost, seeds)
+    with g.Context(0, n_active=64) as ctx:
+        t, st = ctx.solve3d(cost, seeds)
+    assert st["converged"] == 1 and st["tile_w"] == 8
+    _check(t, ref, cost, seeds, ctx.config.min_change)
+
+
+def test_3d_slab_matches_2d(rng):
+    cost2 = rng.uniform(0.1, 10.0, (64, 80)).astype(np.float32)
+    seeds2 = [(3, 4), (70, 50)]
+    print("case 3d slab", flush=True)
+    with g.Context(0) as ctx:
+        t3, _ = ctx.solve3d(cost2[None], [(x, y, 0) for x, y in seeds2])
+        t2, _ = ctx.solve(cost2, seeds2)
+    ref = oracle.fmm_f64(cost2, seeds2)
+    assert np.max(np.abs(t3[0].astype(np.float64) - ref)) <= TOL * ref.max()
+    assert np.max(np.abs(t3[0].astype(np.float64) - t2)) <= 1e-5 * ref.max()
+
+
+@pytest.mark.parametrize("kind", ["speed_f32", "speed_f16", "speed_bf16"])
+def test_3d_speed_kinds(kind):
+    from ghcm_amd import workloads
+    speed = workloads.smooth_speed_3d(48, 3, sigma=4.0)
+    seeds = [(5, 40, 20), (30, 2, 44)]
+    if kind == "speed_f16":
+        stored = speed.astype(np.float16)
+        field = stored
+        eff = stored.astype(np.float32)
+    elif kind == "speed_bf16":
+        field = g.to_bf16(speed)
+        eff = (field.astype(np.uint32) << 16).view(np.float32)
+    else:
+        field = eff = speed
+    cost = (1.0 / eff.astype(np.float64))
+    ref = oracle.fmm3d_f64(cost, seeds)
+    print(f"case 3d kind {kind}", flush=True)
+    with g.Context(0) as ctx:
+        t, _ = ctx.solve3d(field, seeds, kind=g.KINDS[kind])
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(t), fin)
+    assert np.max(np.abs(t[fin] - ref[fin])) <= TOL * ref[fin].max()
+
+
+def test_3d_graph_mode_and_device_entry(rng):
+    cost = rng.uniform(0.5, 2.0, (24, 40, 40)).astype(np.float32)
+    seeds = [(1, 2, 3), (38, 30, 20)]
+    ref = oracle.fmm3d_f64(cost, seeds)
+    print("case 3d graph + device", flush=True)
+    with g.Context(0, launch_mode=2, waves_per_poll=4) as ctx:
+        t_graph, st = ctx.solve3d(cost, seeds)
+        assert st["converged"] == 1
+        dc = ctx.to_device(cost)
+        dt = ctx.alloc(cost.shape, np.float32)
+        ctx.solve3d_device(dc.ptr, g.FIELD_COST_F32, 40, 40, 24, seeds, dt.ptr)
+        t_dev = dt.download()
+    for t in (t_graph, t_dev):
+        assert np.max(np.abs(t - ref)) <= TOL * ref.max()
+
+
+def test_3d_errors():
+    cost = np.ones((4, 4, 4), np.float32)
+    with g.Context(0, scheduler=g.SCHED_HOST_PQ) as ctx:
+        with pytest.raises(g.GhcmError) as e:
+            ctx.solve3d(cost, [(0, 0, 0)])
+        assert e.value.status == g.ERR_UNSUPPORTED
+    with g.Context(0) as ctx:
+        with pytest.raises(g.GhcmError) as e:
+            ctx.solve3d(cost, [(0, 0, 4)])
+        assert e.value.status == g.ERR_INVALID_ARG
+
+
+def test_3d_volume_128_vs_fp64():
+    from ghcm_amd import workloads
+    speed = workloads.smooth_speed_3d(128, 11)
+    cost = (1.0 / speed).astype(np.float32)
+    seeds = [(10, 20, 30), (100, 110, 90), (64, 64, 64)]
+    ref = oracle.fmm3d_f64(cost.astype(np.float64), seeds)
+    print("case 3d volume 128", flush=True)
+    with g.Context(0) as ctx:
+        t, st = ctx.solve3d(cost, seeds)
+    assert st["converged"] == 1
+    _check(t, ref, cost, seeds, ctx.config.min_change)
