@@ -665,6 +665,10 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         o.cc = at(o.b, CB + PW + 1);
         r = (r + step_dir) & (TH - 1);
     };
+    // Destination of the DPP move, carried from step to step: lane 0 has no source lane, is
+    // never written and keeps +INF (its left neighbour is the halo, read from LDS as n.tl),
+    // so no copy of an `old` operand is needed per step.
+    float lreg = __builtin_inff();
     for (;;) {
         float prev = __builtin_inff();
         float maxdec = 0.0f;
@@ -675,9 +679,9 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
         };
         auto step = [&](const Nb& n) {
-            // Lane L-1's fresh value of the left cell (lane 0: the LDS value, its halo).
-            const float left = dpp_from_lower_lane(n.tl, prev);
-            const float ax = umin(umin(n.tl, left), n.tr);
+            // Lane L-1's fresh value of the left cell.
+            lreg = dpp_from_lower_lane(lreg, prev);
+            const float ax = umin(umin(n.tl, lreg), n.tr);
             const float up = n.first ? __builtin_inff() : prev;
             const float ay = umin(umin(n.tu, up), n.td);
             const float t = godunov_raw(ax, ay, umin(ax, ay), n.cc);
