@@ -688,13 +688,18 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             // Merge into the cell with an LDS min (no return value is waited for): the four
             // sweeps write the same tile concurrently and a min never raises a cell another
             // sweep has just lowered.  Non-negative floats order like their bit patterns.
-            const float nv = umin(t, n.tc);
             if (TW == 64 || lane_on)
                 __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * (PW + 1)),
                                        __float_as_uint(t), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-            maxdec = fmaxf(maxdec, n.tc - nv);       // NaN (both +INF) is ignored by fmaxf
-            prev = nv;
+            // Largest decrease (negative when t is not below the old value; NaN, from two
+            // +INF minima, is ignored by fmaxf).
+            maxdec = fmaxf(maxdec, n.tc - t);
+            // The raw result is handed on, not min(t, old): both consumers (lane L+1 next step
+            // as its left neighbour, this lane next step as its upper one) take the minimum
+            // with their own LDS read of the cell, issued after this lane's read of it in the
+            // same wave and therefore no larger than the old value seen here.
+            prev = t;
         };
         Nb A, B;
         load(A);
