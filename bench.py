@@ -403,7 +403,7 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the gHCM path is a single-"
-              "device solve; multi-GPU sharding is described in DESIGN.md). Use --gpus 1.",
+              "device solve; sharding across GPUs is designed but not built). Use --gpus 1.",
               file=sys.stderr)
         return 2
 

@@ -2,7 +2,7 @@
 
 - generator_fingerprints.json: digests of fixed-seed workload fields, so the seeded
   generators of ghcm_amd/csrc/field_gen.cpp cannot drift silently.
-- c1_ring256_fmm64.npz: the 256x256 seven-ring workload (seed 1, walls at speed 1e-3,
+- ring256_fmm64.npz: the 256x256 seven-ring workload (seed 1, walls at speed 1e-3,
   centre source) and its FP64 FMM solution from the oracle, stored as float32.
 - hand_kats.json: small grids whose solutions are derived by hand from the closed forms
   (1-D strips: prefix sums; constant cost from a corner: k*c on the axes,
@@ -48,7 +48,7 @@ def main():
         json.dump(fingerprints(), f, indent=1)
     speed = g.gen_ring_speed(256, 256, 1, 1e-3, 1.0)
     t = oracle.fmm_f64(1.0 / speed, [(128, 128)])
-    np.savez_compressed(os.path.join(HERE, "c1_ring256_fmm64.npz"),
+    np.savez_compressed(os.path.join(HERE, "ring256_fmm64.npz"),
                         speed=speed.astype(np.float32), time=t.astype(np.float32),
                         seeds=np.array([[128, 128]], dtype=np.uint32))
     kats = {

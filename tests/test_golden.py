@@ -41,8 +41,8 @@ def test_hand_kats(name):
             assert t[y, x] == pytest.approx(val, rel=1e-6), (sname, key)
 
 
-def test_c1_fixture_regression():
-    d = np.load(os.path.join(GOLDEN, "c1_ring256_fmm64.npz"))
+def test_ring256_fixture_regression():
+    d = np.load(os.path.join(GOLDEN, "ring256_fmm64.npz"))
     speed = g.gen_ring_speed(256, 256, 1, 1e-3, 1.0)
     assert np.array_equal(speed.astype(np.float32), d["speed"])
     seeds = d["seeds"]

@@ -38,8 +38,8 @@ def check(t, ref, seeds, cost=None, tol=TOL, label="", eps=1e-5):
 
 
 @pytest.fixture(scope="module")
-def c1():
-    d = np.load(os.path.join(GOLDEN, "c1_ring256_fmm64.npz"))
+def rings256():
+    d = np.load(os.path.join(GOLDEN, "ring256_fmm64.npz"))
     speed = g.gen_ring_speed(256, 256, 1, 1e-3, 1.0)
     cost = g.cost_from_speed(speed)
     return cost, d["seeds"], oracle.fmm_f32(cost, d["seeds"]), d["time"]
@@ -48,9 +48,9 @@ def c1():
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("tile", TILES)
 @pytest.mark.parametrize("sched", [g.SCHED_DEVICE_TOPN, g.SCHED_HOST_PQ])
-def test_c1_rings_every_config(c1, kernel, tile, sched):
-    cost, seeds, ref32, ref64 = c1
-    print(f"case c1 kernel={kernel} tile={tile} sched={sched}", flush=True)
+def test_rings256_every_config(rings256, kernel, tile, sched):
+    cost, seeds, ref32, ref64 = rings256
+    print(f"case rings256 kernel={kernel} tile={tile} sched={sched}", flush=True)
     with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, scheduler=sched,
                    compute_unit_overload=4.0) as ctx:
         t, st = ctx.solve(cost, seeds)
@@ -64,9 +64,9 @@ def test_c1_rings_every_config(c1, kernel, tile, sched):
                                                (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
                                                (g.SCHED_HOST_PQ, 0)])
 @pytest.mark.parametrize("cap", [0, 1])
-def test_c1_rings_sweep_kernel(c1, tile, sched, launch_mode, cap):
-    cost, seeds, ref32, ref64 = c1
-    print(f"case c1 sweep tile={tile} sched={sched} launch_mode={launch_mode} cap={cap}", flush=True)
+def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
+    cost, seeds, ref32, ref64 = rings256
+    print(f"case rings256 sweep tile={tile} sched={sched} launch_mode={launch_mode} cap={cap}", flush=True)
     with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=g.KERNEL_SWEEP, scheduler=sched,
                    launch_mode=launch_mode, inner_iter_cap=cap, n_active=64) as ctx:
         t, st = ctx.solve(cost, seeds)
@@ -76,9 +76,9 @@ def test_c1_rings_sweep_kernel(c1, tile, sched, launch_mode, cap):
 
 
 @pytest.mark.parametrize("n_active", [1, 3, 64, 100000])
-def test_top_n_sizes(c1, n_active):
+def test_top_n_sizes(rings256, n_active):
     """n smaller than the active set exercises the device top-n selection."""
-    cost, seeds, ref32, _ = c1
+    cost, seeds, ref32, _ = rings256
     with g.Context(0, n_active=n_active) as ctx:
         t, st = ctx.solve(cost, seeds)
     check(t, ref32, seeds, label=f"n={n_active}")
@@ -143,8 +143,8 @@ def test_strip_prefix_sums():
     assert np.allclose(t[0], exp, rtol=1e-5)
 
 
-def test_epsilon_zero_exact_convergence(c1):
-    cost, seeds, ref32, _ = c1
+def test_epsilon_zero_exact_convergence(rings256):
+    cost, seeds, ref32, _ = rings256
     for kernel in KERNELS + [g.KERNEL_SWEEP]:
         tile = (64, 64) if kernel == g.KERNEL_SWEEP else (8, 8)
         with g.Context(0, min_change=0.0, kernel=kernel, tile_w=tile[0], tile_h=tile[1]) as ctx:
@@ -171,8 +171,8 @@ def test_metamorphic_configs_agree(rng):
         assert np.max(np.abs(t - outs[0])) <= TOL * ref.max()
 
 
-def test_cxx_api_path(c1):
-    cost, seeds, ref32, _ = c1
+def test_cxx_api_path(rings256):
+    cost, seeds, ref32, _ = rings256
     for kernel in KERNELS:
         t, st = g.hcm_solve_cxx(cost, seeds, block_size=16, compute_unit_overload=2.0,
                                 kernel=kernel)
@@ -194,6 +194,10 @@ def test_errors_reported():
         ctx.set_config(max_waves=0, waves_per_poll=0)
         t, _ = ctx.solve(cost, [(0, 0)])           # context still usable
         assert t[0, 0] == 0.0
+        # Widths above GHCM_MAX_WIDTH (4,194,304) are refused before any buffer is touched.
+        with pytest.raises(g.GhcmError) as e:
+            ctx.solve_device(0x1000, g.FIELD_COST_F32, 4194305, 1, [(0, 0)], 0x1000)
+        assert e.value.status == g.ERR_INVALID_ARG and "width" in str(e.value)
 
 
 def test_context_reuse_after_smaller_and_larger():
