@@ -1048,7 +1048,7 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
     return (int)cur;
 }
 
-constexpr int kSelectStage = 4096;     // list entries staged in LDS by k_wave_end
+constexpr int kSelectStage = 16384;    // list entries staged in LDS by k_wave_end (128 KB)
 
 // Serving: queued tiles per instance slot in the new current list `cur` (prefix in
 // s_prefix), tagged with the slot's admission generation; a count of 0 means nothing is
