@@ -499,7 +499,9 @@ def main() -> int:
             f"{out['batched']['speedup_vs_plain']}x plain")
     if not args.no_serve:
         log("bench: serving")
-        scfg = dict(plain_cfg)
+        # A serving loop runs many requests per wave: without the batch autotuner's choice it
+        # selects every queued tile (n = 65,536), as the autotuner does for the batch.
+        scfg = dict(plain_cfg, n_active=65536)
         if "autotuned" in out.get("batched", {}):
             scfg.update(out["batched"]["autotuned"])
         out["serving"] = bench_serving(g, scfg)
