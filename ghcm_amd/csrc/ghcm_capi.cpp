@@ -271,8 +271,13 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
     const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
     if (three_d && host_sched)
         return fail(GHCM_ERR_UNSUPPORTED, "3-D solves run with the device scheduler only");
+    uint32_t n_sel = cfg.n_active ? cfg.n_active
+                                  : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * ctx->cus));
+    // Waves far wider than the chip (batches of many instances) take the higher-occupancy
+    // form of the tile kernel where one is built.
+    const bool wide = (uint64_t)n_sel >= 4ull * (uint64_t)std::max(1, ctx->cus);
     const SolveKernel sk = three_d ? get_solve3d_kernel(kind)
-                                   : get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched);
+                                   : get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched, wide);
     const solve_launcher_t launch = sk.launch;
     if (!launch)
         return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u kind %d kernel %d%s", tw, th,
@@ -344,8 +349,6 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
                 hs[k++] = i;
             }
     }
-    uint32_t n_sel = cfg.n_active ? cfg.n_active
-                                  : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * ctx->cus));
     if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, std::min<uint64_t>(n_sel, T))))) return rc;
     // Grid of the device-mode solve kernel: at most one resident round of work-groups;
     // each work-group loops over list entries, so the grid never has to match the list.

@@ -17,8 +17,10 @@ struct SolveKernel {
     int (*blocks_per_cu)();           // resident work-groups per CU (occupancy)
 };
 
-// mode: 0 systolic, 1 FIM, 2 sweep.  host: job list given by the host scheduler.
-SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host);
+// mode: 0 systolic, 1 FIM, 2 sweep.  host: job list given by the host scheduler.  wide: the
+// waves hold many more tiles than CUs (selects a higher-occupancy form where one is built).
+SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host,
+                             bool wide = false);
 // Three-dimensional tile solve (8 x 8 x 8 tiles, device scheduler) for a field kind.
 SolveKernel get_solve3d_kernel(int kind);
 constexpr uint32_t kTile3D = 8;

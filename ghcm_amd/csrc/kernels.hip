@@ -824,8 +824,13 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     }
 }
 
-template <int TW, int TH, int KIND, bool HOST>
-__global__ __launch_bounds__(kSweepThreads) void k_solve_sweep(SolveArgs a) {
+// WPE: minimum waves per SIMD the register allocation must allow (1: no constraint).  The
+// 4-wave form (at most 128 VGPRs, four work-groups per CU) serves waves with many more tiles
+// than CUs (batches, serving), where occupancy hides latency; the 1-wave form keeps the
+// shorter single-tile latency that a plain solve's waves of a few hundred tiles are bound by.
+template <int TW, int TH, int KIND, bool HOST, int WPE = 1>
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_solve_sweep(SolveArgs a) {
     __shared__ float sh[sweep_lds_words<TW, TH>()];
     __shared__ uint32_t s_u32[16];
     __shared__ float s_f32[8];
@@ -1732,24 +1737,24 @@ static int occ_solve_t() {
     return nb > 0 ? nb : 1;
 }
 
-template <int TW, int TH, int KIND, bool HOST>
+template <int TW, int TH, int KIND, bool HOST, int WPE = 1>
 static int occ_sweep_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep<TW, TH, KIND, HOST>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep<TW, TH, KIND, HOST, WPE>,
                                                      kSweepThreads, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
-template <int TW, int TH, int KIND, bool HOST>
+template <int TW, int TH, int KIND, bool HOST, int WPE = 1>
 static hipError_t launch_sweep_t(const SolveArgs& a, uint32_t grid, hipStream_t s,
                                  hipEvent_t ev0, hipEvent_t ev1) {
     if (ev0 && ev1)
-        hipExtLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST>), dim3(grid), dim3(kSweepThreads),
-                              0, s, ev0, ev1, 0, a);
+        hipExtLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST, WPE>), dim3(grid),
+                              dim3(kSweepThreads), 0, s, ev0, ev1, 0, a);
     else
-        hipLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST>), dim3(grid), dim3(kSweepThreads), 0,
-                           s, a);
+        hipLaunchKernelGGL((k_solve_sweep<TW, TH, KIND, HOST, WPE>), dim3(grid),
+                           dim3(kSweepThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -1865,8 +1870,21 @@ static SolveKernel pick_mode(int kind, int mode, bool host) {
     return {nullptr, nullptr};
 }
 
-SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host) {
+// The 4-waves-per-SIMD form of the 64x64 sweep kernel, for device-scheduled waves wide
+// enough to fill four work-groups per CU.
+static SolveKernel pick_sweep_wide(int kind) {
+    switch (kind) {
+        case 0: return {&launch_sweep_t<64, 64, 0, false, 4>, &occ_sweep_t<64, 64, 0, false, 4>};
+        case 1: return {&launch_sweep_t<64, 64, 1, false, 4>, &occ_sweep_t<64, 64, 1, false, 4>};
+        case 2: return {&launch_sweep_t<64, 64, 2, false, 4>, &occ_sweep_t<64, 64, 2, false, 4>};
+        case 3: return {&launch_sweep_t<64, 64, 3, false, 4>, &occ_sweep_t<64, 64, 3, false, 4>};
+    }
+    return {nullptr, nullptr};
+}
+
+SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host, bool wide) {
     if (mode == 2) {
+        if (wide && !host && tw == 64 && th == 64) return pick_sweep_wide(kind);
         if (tw == 64 && th == 128) return pick_sweep<64, 128>(kind, host);
         if (tw == 64 && th == 64) return pick_sweep<64, 64>(kind, host);
         if (tw == 64 && th == 32) return pick_sweep<64, 32>(kind, host);

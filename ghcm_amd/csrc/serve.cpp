@@ -254,13 +254,16 @@ int ghcm_server::init() {
     HIP_TRY(hipStreamCreateWithFlags(&down_stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&ev_upload, hipEventDisableTiming));
 
-    const SolveKernel sk = get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, false);
+    n_sel = cfg.n_active ? cfg.n_active
+                         : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * cus));
+    // Waves far wider than the chip (the usual case for a serving loop) take the higher-
+    // occupancy form of the tile kernel where one is built.
+    const SolveKernel sk = get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, false,
+                                            (uint64_t)n_sel >= 4ull * (uint64_t)cus);
     launch = sk.launch;
     if (!launch)
         return fail(GHCM_ERR_UNSUPPORTED, "no kernel for tile %ux%u kind %d kernel %d", tw, th, kind,
                     cfg.kernel);
-    n_sel = cfg.n_active ? cfg.n_active
-                         : (uint32_t)std::max(1.0, std::floor((double)cfg.compute_unit_overload * cus));
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, cus);
     grid = std::max(1u, std::min(n_sel, resident));
     lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);

@@ -75,13 +75,16 @@ def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     check(t, ref64.astype(np.float64), seeds, label="sweep vs fmm64 fixture")
 
 
+@pytest.mark.parametrize("sweep", [False, True])
 @pytest.mark.parametrize("n_active", [1, 3, 64, 100000])
-def test_top_n_sizes(rings256, n_active):
-    """n smaller than the active set exercises the device top-n selection."""
+def test_top_n_sizes(rings256, n_active, sweep):
+    """n smaller than the active set exercises the device top-n selection; with the 64x64
+    sweep kernel, n far above the CU count selects its higher-occupancy form."""
     cost, seeds, ref32, _ = rings256
-    with g.Context(0, n_active=n_active) as ctx:
+    cfg = dict(tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, inner_iter_cap=1) if sweep else {}
+    with g.Context(0, n_active=n_active, **cfg) as ctx:
         t, st = ctx.solve(cost, seeds)
-    check(t, ref32, seeds, label=f"n={n_active}")
+    check(t, ref32, seeds, cost, label=f"n={n_active} sweep={sweep}")
 
 
 SHAPES = [(1, 1), (1, 37), (37, 1), (13, 29), (100, 70), (3, 250), (257, 129)]
