@@ -1006,7 +1006,10 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
             __syncthreads();
         }
         // Compaction: keys < lo_base (there are `below` of them) all go; keys in
-        // [lo_base, lo_base + 128) go while the quota n - below lasts.
+        // [lo_base, lo_base + 128) go while the quota n - below lasts.  The histogram is free
+        // from here on; its first kShards words count the carried entries per shard.
+        if (tid < (int)kShards) hist[tid] = 0u;
+        __syncthreads();
         const uint32_t quota = n - below;
         uint32_t quota_used = 0;
         const uint32_t thr_hi = lo_base + (1u << 7);
@@ -1028,14 +1031,24 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
                 if (sel) {
                     a.jobs[base + pos] = g;
                 } else {
-                    // carry: move the activation record to the next parity, re-queue
+                    // Carry: move the activation record to the next parity and queue the tile
+                    // in list cur^1 for the wave after.  That list was emptied above (or is
+                    // still empty on the first boundary) and nothing else runs during the
+                    // boundary, so the tile is not in it (qwave < wave + 2), its shard slot
+                    // comes from an LDS counter and qwave is a plain store: no returned
+                    // global atomic sits on this path (activate() needs two in a row).
                     uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
                     uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
                     const uint32_t mb = *mp;
-                    const uint32_t kk = *kp;
                     *kp = kInfBits;
                     *mp = 0u;
-                    activate(a, g, wave, cur, p, mb, __uint_as_float(kk));
+                    const size_t np = p ^ 1u;
+                    atomicMin(&a.key[np * a.tiles + g], k);
+                    atomicOr(&a.mask[np * a.tiles + g], mb);
+                    a.qwave[g] = wave + 2u;
+                    const uint32_t sh = g % kShards;
+                    const uint32_t slot = atomicAdd(&hist[sh], 1u);
+                    a.list[((size_t)(cur ^ 1u) * kShards + sh) * a.shard_cap + slot] = g;
                 }
             }
             base += tot_sel;
@@ -1045,6 +1058,8 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
     };
     if (staged) passes(std::true_type{});
     else passes(std::false_type{});
+    __syncthreads();
+    if (tid < (int)kShards && hist[tid]) ctrl->count[cur ^ 1u][tid * kCounterStride] = hist[tid];
     if (tid == 0) {
         ctrl->jobs_from_list = 0u;
         ctrl->n_jobs = base;
