@@ -389,7 +389,7 @@ def main() -> int:
     ap.add_argument("--kernel", default="sweep", choices=["systolic", "fim", "sweep"])
     ap.add_argument("--n-active", type=int, default=512)
     ap.add_argument("--inner-cap", type=int, default=1, help="in-tile rounds per visit (0 = converge)")
-    ap.add_argument("--waves-per-poll", type=int, default=8)
+    ap.add_argument("--waves-per-poll", type=int, default=4)
     ap.add_argument("--launch-mode", type=int, default=0, help="0 per-wave launches, 1 persistent")
     ap.add_argument("--cpu-trials", type=int, default=5)
     ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision report")

@@ -459,7 +459,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         a.rec = ctx->rec.dev_as<HostRec>();
         HIP_TRY(launch_wave_end(a, 1u, st));
         const bool prof = cfg.profile_kernels != 0;
-        const uint32_t lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);
+        const uint32_t lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 4);
         const bool use_graph = cfg.launch_mode == 2 && !prof;
         // The fused kernel's grid: one resident round of its (larger-LDS) work-groups.
         const uint32_t fgrid = fused ? std::max(1u, std::min(n_sel, (uint32_t)std::max(1, fk.blocks_per_cu()) *

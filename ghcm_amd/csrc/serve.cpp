@@ -266,7 +266,7 @@ int ghcm_server::init() {
                     cfg.kernel);
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, cus);
     grid = std::max(1u, std::min(n_sel, resident));
-    lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 8);
+    lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 4);
     step = cfg.launch_mode == 2 ? std::max<uint32_t>(1, lead / 2) : 1;
 
     const uint64_t T = max_tiles;

@@ -89,8 +89,9 @@ typedef struct ghcm_config {
     int32_t  scheduler;             /* ghcm_scheduler                                 */
     uint32_t inner_iter_cap;        /* in-tile passes per visit, 0 = until converged  */
     uint32_t max_waves;             /* safety cap on waves, 0 = automatic             */
-    uint32_t waves_per_poll;        /* device scheduler: waves per enqueued chunk (4
-                                       chunks in flight between polls), 0 = 4         */
+    uint32_t waves_per_poll;        /* device scheduler: waves kept enqueued ahead of
+                                       the last executed wave boundary (at least 2),
+                                       0 = 4                                          */
     uint32_t profile_kernels;       /* 1: time every tile-solve launch with HIP events
                                        (fills solve_kernel_us / solve_launches)       */
     uint32_t launch_mode;           /* device scheduler: 0 = one tile-solve launch per
