@@ -723,6 +723,26 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         if (rounds >= cap) { unfinished = true; break; }
     }
 
+    // Commit improved cells first (plain stores: one writer per tile per wave), so the stores
+    // drain while the border check and the activations run; the tile min is reduced after.
+    char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
+    float tmin = __builtin_inff();
+    uint32_t improved = 0;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int i = tid + kSweepThreads * k;
+        if (i < P) {
+            const int cx = i % TW, cy = i / TW;
+            if (cx < sx && cy < sy) {
+                const float tn = t_s[(cy + 1) * PW + cx + 1];
+                tmin = fminf(tmin, tn);
+                if (tn < told[k]) {
+                    *gmem(reinterpret_cast<float*>(cb + ((uint32_t)cy * (uint32_t)W + (uint32_t)cx) * 4u)) = tn;
+                    ++improved;
+                }
+            }
+        }
+    }
     // Border check, wave w takes side w: would the neighbour's adjacent cell improve by
     // >= eps from this tile's edge (data/eikonal_ghcm_systolic.cl:8-41)?
     {
@@ -773,25 +793,6 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         }
     }
 
-    // Commit improved cells (plain stores: one writer per tile per wave), tile min.
-    char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
-    float tmin = __builtin_inff();
-    uint32_t improved = 0;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const int i = tid + kSweepThreads * k;
-        if (i < P) {
-            const int cx = i % TW, cy = i / TW;
-            if (cx < sx && cy < sy) {
-                const float tn = t_s[(cy + 1) * PW + cx + 1];
-                tmin = fminf(tmin, tn);
-                if (tn < told[k]) {
-                    *gmem(reinterpret_cast<float*>(cb + ((uint32_t)cy * (uint32_t)W + (uint32_t)cx) * 4u)) = tn;
-                    ++improved;
-                }
-            }
-        }
-    }
     tmin = wave_min(tmin);
     improved = wave_sum(improved);
     if (L == 0) {
