@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 SWEEP = dict(tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512, inner_iter_cap=1,
              waves_per_poll=8)
 SYSTOLIC = dict(tile_w=16, tile_h=16, kernel=g.KERNEL_SYSTOLIC, n_active=2048, waves_per_poll=8)
+# n far above the CU count: the server runs the higher-occupancy form of the sweep kernel.
+SWEEP_WIDE = dict(SWEEP, n_active=65536, waves_per_poll=4)
 
 
 @pytest.fixture(scope="module")
@@ -34,7 +36,7 @@ def _serve_all(server, reqs, order=None):
     return {i: server.wait(t, 60.0) for i, t in tickets.items()}
 
 
-@pytest.mark.parametrize("cfg", [SWEEP, SYSTOLIC], ids=["sweep64", "systolic16"])
+@pytest.mark.parametrize("cfg", [SWEEP, SWEEP_WIDE, SYSTOLIC], ids=["sweep64", "sweep64wide", "systolic16"])
 def test_alone_equals_mix_and_oracle(mix, cfg):
     print(f"case serve alone-vs-mix cfg={cfg}", flush=True)
     with g.Server(0, **cfg) as srv:
