@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <cstddef>
 
@@ -101,6 +103,9 @@ struct PinnedBuf {
             p = dev = nullptr;
             return fail(GHCM_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", b, hipGetErrorString(e));
         }
+        // Pinned memory comes back from the runtime with whatever a previous owner left in
+        // it; host records are read as tags and progress words, so start from zero.
+        std::memset(p, 0, b);
         bytes = b;
         return GHCM_OK;
     }

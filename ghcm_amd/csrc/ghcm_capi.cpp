@@ -359,6 +359,7 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
         max_cells = std::max<uint64_t>(max_cells, (uint64_t)hi[i].w * hi[i].h * (three_d ? hi[i].d : 1u));
     volatile HostRec* hr = ctx->rec.as<HostRec>();
     hr->progress = 0ull;
+    hr->totals_tag = 0u;      // this solve's tag is written by its own k_publish_totals
 
     SolveArgs a{};
     a.inst = ctx->inst.as<GhcmInst>();

@@ -203,6 +203,19 @@ def test_errors_reported():
         assert e.value.status == g.ERR_INVALID_ARG and "width" in str(e.value)
 
 
+def test_stats_of_a_new_context_are_its_own(rings256):
+    """A context's first solve reports its own counters, not a previous context's (the host
+    record lives in pinned memory that the runtime may hand out again)."""
+    cost, seeds, ref32, _ = rings256
+    with g.Context(0, compute_unit_overload=4.0) as ctx:
+        _, st_small = ctx.solve(cost, seeds)
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, inner_iter_cap=1) as ctx:
+        _, st_big = ctx.solve(cost, seeds)
+    # 8x8 tiles: 1,024 tiles, thousands of visits; 64x64 tiles: 16 tiles
+    assert st_small["tile_visits"] > 1000
+    assert 0 < st_big["tile_visits"] < 1000, st_big
+
+
 def test_context_reuse_after_smaller_and_larger():
     with g.Context(0, tile_w=16, tile_h=16) as ctx:
         for (h, w) in ((300, 200), (17, 33), (512, 700)):
