@@ -449,6 +449,14 @@ def main() -> int:
     log(f"bench: {args.steps} steps, {ms_per_step:.3f} ms/step, {value:.1f} Mcells/s")
 
     t_gpu = d_time.download()
+    # The same solve through the host-buffer entry point (field in, times out over PCIe):
+    # reported beside `value`, never as it.
+    hts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ctx.solve(cost, seeds)
+        hts.append(time.perf_counter() - t0)
+    host_ms = statistics.median(hts) * 1e3
     # Roofline of the tile-solve kernel over the same number of instrumented steps.
     ctx.set_config(profile_kernels=1)
     psts = [step() for _ in range(args.steps)]
@@ -487,7 +495,11 @@ def main() -> int:
                      "inner_passes_per_visit": round(sts[0]["inner_iterations"] /
                                                      max(1, sts[0]["tile_visits"]), 2),
                      "device_ms": round(statistics.median(s["duration_device_us"] for s in sts) / 1e3, 4)},
-           "roofline": roof}
+           "roofline": roof,
+           "host_buffers": {"ms_per_solve": round(host_ms, 3),
+                            "mcells_per_s": round(W * H / (host_ms / 1e3) / 1e6, 1),
+                            "note": "ghcm_solve with the field and times in host memory "
+                                    "(128 MB over PCIe per solve)"}}
 
     plain_cfg = dict(tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
                      inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,

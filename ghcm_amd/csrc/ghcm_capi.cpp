@@ -65,6 +65,8 @@ struct ghcm_ctx {
     // device scratch
     DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals, sel;
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
+    PinnedBuf io_stage[2];      // pinned parts of host-buffer copies (ping-pong)
+    hipEvent_t ev_io[2] = {nullptr, nullptr};
     PinnedBuf h_ctrl, h_seeds, h_inst, h_jobs, h_status;
     PinnedBuf rec;                       // HostRec written by the wave loop
     uint32_t solve_seq = 0;              // tags the totals of each solve
@@ -212,8 +214,10 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
                       &ctx->field_io, &ctx->time_io})
         b->release();
     for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
-                         &ctx->h_status, &ctx->rec})
+                         &ctx->h_status, &ctx->rec, &ctx->io_stage[0], &ctx->io_stage[1]})
         b->release();
+    for (hipEvent_t ev : ctx->ev_io)
+        if (ev) (void)hipEventDestroy(ev);
     if (ctx->ev_begin) (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) (void)hipEventDestroy(ctx->ev_end);
     if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
@@ -691,6 +695,53 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
 // solve entry points
 // ---------------------------------------------------------------------------------------
 
+// Host-buffer copies go through two 16 MB pinned parts: the host copy of one part (split over
+// threads) overlaps the DMA of the other.  A pageable hipMemcpyAsync of the 4096^2 field and
+// times moved about 14 GB/s.
+constexpr size_t kIoPart = size_t(16) << 20;
+
+int io_prepare(ghcm_ctx* ctx) {
+    for (int b = 0; b < 2; ++b) {
+        if (int rc = ctx->io_stage[b].reserve(kIoPart)) return rc;
+        if (!ctx->ev_io[b]) HIP_TRY(hipEventCreateWithFlags(&ctx->ev_io[b], hipEventDisableTiming));
+    }
+    return GHCM_OK;
+}
+
+int stage_in(ghcm_ctx* ctx, void* d_dst, const void* h_src, size_t n) {
+    if (int rc = io_prepare(ctx)) return rc;
+    int b = 0;
+    for (size_t off = 0; off < n; off += kIoPart, b ^= 1) {
+        const size_t len = std::min(kIoPart, n - off);
+        HIP_TRY(ghcm_capi::spin_wait(ctx->ev_io[b]));       // part b's previous DMA is done
+        ghcm_capi::par_memcpy(ctx->io_stage[b].p, static_cast<const char*>(h_src) + off, len);
+        HIP_TRY(hipMemcpyAsync(static_cast<char*>(d_dst) + off, ctx->io_stage[b].p, len,
+                               hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev_io[b], ctx->stream));
+    }
+    return GHCM_OK;
+}
+
+int stage_out(ghcm_ctx* ctx, void* h_dst, const void* d_src, size_t n) {
+    if (int rc = io_prepare(ctx)) return rc;
+    const size_t parts = (n + kIoPart - 1) / kIoPart;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t off = i * kIoPart, len = std::min(kIoPart, n - off);
+        hipError_t e = hipMemcpyAsync(ctx->io_stage[i & 1].p, static_cast<const char*>(d_src) + off,
+                                      len, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev_io[i & 1], ctx->stream);
+        return e;
+    };
+    for (size_t i = 0; i < parts && i < 2; ++i) HIP_TRY(issue(i));
+    for (size_t i = 0; i < parts; ++i) {
+        const size_t off = i * kIoPart, len = std::min(kIoPart, n - off);
+        HIP_TRY(ghcm_capi::spin_wait(ctx->ev_io[i & 1]));
+        ghcm_capi::par_memcpy(static_cast<char*>(h_dst) + off, ctx->io_stage[i & 1].p, len);
+        if (i + 2 < parts) HIP_TRY(issue(i + 2));
+    }
+    return GHCM_OK;
+}
+
 extern "C" int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
                                  uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
                                  float* d_time, ghcm_stats* stats) {
@@ -712,12 +763,11 @@ extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint
     int rc = ctx->field_io.reserve(n * kind_bytes(field_kind));
     if (rc) return rc;
     if ((rc = ctx->time_io.reserve(n * 4))) return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->field_io.p, field, n * kind_bytes(field_kind), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = stage_in(ctx, ctx->field_io.p, field, n * kind_bytes(field_kind)))) return rc;
     HostInst hi{ctx->field_io.p, ctx->time_io.as<float>(), w, h, seeds_xy, n_seeds};
     rc = solve_core(ctx, field_kind, &hi, 1, stats);
     if (rc != GHCM_OK && rc != GHCM_ERR_NOT_CONVERGED) return rc;
-    HIP_TRY(hipMemcpyAsync(time_out, ctx->time_io.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (int rc2 = stage_out(ctx, time_out, ctx->time_io.p, n * 4)) return rc2;
     if (stats)
         stats->duration_total_us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
@@ -745,12 +795,11 @@ extern "C" int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, ui
     int rc = ctx->field_io.reserve(n * kind_bytes(field_kind));
     if (rc) return rc;
     if ((rc = ctx->time_io.reserve(n * 4))) return rc;
-    HIP_TRY(hipMemcpyAsync(ctx->field_io.p, field, n * kind_bytes(field_kind), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = stage_in(ctx, ctx->field_io.p, field, n * kind_bytes(field_kind)))) return rc;
     HostInst hi{ctx->field_io.p, ctx->time_io.as<float>(), w, h, seeds_xyz, n_seeds, d};
     rc = solve_core(ctx, field_kind, &hi, 1, stats, true);
     if (rc != GHCM_OK && rc != GHCM_ERR_NOT_CONVERGED) return rc;
-    HIP_TRY(hipMemcpyAsync(time_out, ctx->time_io.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (int rc2 = stage_out(ctx, time_out, ctx->time_io.p, n * 4)) return rc2;
     if (stats)
         stats->duration_total_us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();

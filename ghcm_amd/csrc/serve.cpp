@@ -44,6 +44,7 @@
 using namespace ghcm;
 using ghcm_capi::DevBuf;
 using ghcm_capi::PinnedBuf;
+using ghcm_capi::par_memcpy;
 using ghcm_capi::fail;
 using ghcm_capi::kind_bytes;
 
@@ -762,35 +763,6 @@ int enqueue_request(ghcm_server* s, Request* r, uint64_t* ticket) {
     s->cv_work.notify_one();
     *ticket = r->ticket;
     return GHCM_OK;
-}
-
-// Host copy into or out of pinned staging.  One core's memcpy bandwidth (about 8-10 GB/s)
-// is far below the host's, and host-buffer requests of a few MB to 64 MB are bound by it, so
-// copies of 8 MB and more are split over up to 8 threads in 4 MB-aligned parts.  If a thread
-// cannot be started, the caller copies that part itself.
-void par_memcpy(void* dst, const void* src, size_t n) {
-    constexpr size_t kPart = size_t(4) << 20;
-    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-    const size_t parts = std::min<size_t>(std::min<size_t>(n / kPart, 8), hw);
-    if (parts < 2) {
-        std::memcpy(dst, src, n);
-        return;
-    }
-    const size_t per = ((n + parts - 1) / parts + kPart - 1) / kPart * kPart;
-    auto* d = static_cast<char*>(dst);
-    const auto* sp = static_cast<const char*>(src);
-    std::vector<std::thread> th;
-    th.reserve(parts);
-    for (size_t off = per; off < n; off += per) {
-        const size_t len = std::min(per, n - off);
-        try {
-            th.emplace_back([d, sp, off, len] { std::memcpy(d + off, sp + off, len); });
-        } catch (...) {
-            std::memcpy(d + off, sp + off, len);
-        }
-    }
-    std::memcpy(d, sp, std::min(per, n));
-    for (auto& t : th) t.join();
 }
 
 }  // namespace
