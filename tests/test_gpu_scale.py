@@ -41,6 +41,29 @@ def test_plain4096_full_size_parity(plain4096):
             assert t[y, x] == 0.0
 
 
+def test_plain4096_host_buffers(plain4096):
+    """The host-buffer entry point stages the 64 MB field and the 64 MB result through
+    pinned parts (several per direction at this size); it must hand back what the device
+    path computes, and the oracle's answer."""
+    speed, cost, seeds = plain4096
+    print("case plain4096 host buffers", flush=True)
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
+                   inner_iter_cap=1) as ctx:
+        t_host, st = ctx.solve(cost, seeds)
+        # speed kind: the FP32 speed field (also 64 MB) through the same staging
+        t_speed, _ = ctx.solve(speed.astype(np.float32), seeds, kind=g.FIELD_SPEED_F32)
+        dc = ctx.to_device(cost)
+        dt = ctx.alloc(cost.shape, np.float32)
+        ctx.solve_device(dc.ptr, g.FIELD_COST_F32, 4096, 4096, seeds, dt.ptr)
+        t_dev = dt.download()
+    assert st["converged"] == 1
+    ref = oracle.fmm_f32(cost, seeds)
+    for t in (t_host, t_speed):
+        assert np.all(np.isfinite(t))
+        assert np.max(np.abs(t.astype(np.float64) - ref)) <= 1e-4 * ref.max()
+        assert np.max(np.abs(t.astype(np.float64) - t_dev)) <= 1e-5 * ref.max()
+
+
 @pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
 def test_precision_storage_vs_fp64(kind, tol):
     """Speed stored as FP32/FP16/BF16, times and updates in FP32, against the FP64 FMM."""
