@@ -90,34 +90,11 @@ struct DevBuf {
 
 // Pinned host buffer that only grows: mapped into the device's address space and
 // coherent, so kernels read and write it directly (`dev` is the device view).
-// Host copy into or out of pinned staging.  One core's memcpy bandwidth (about 8-10 GB/s)
-// is far below the host's, and host-buffer requests of a few MB to 64 MB are bound by it, so
-// copies of 4 MB and more are split over up to 8 threads in 2 MB-aligned parts.  If a thread
-// cannot be started, the caller copies that part itself.
-inline void par_memcpy(void* dst, const void* src, size_t n) {
-    constexpr size_t kPart = size_t(2) << 20;
-    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
-    const size_t parts = std::min<size_t>(std::min<size_t>(n / kPart, 8), hw);
-    if (parts < 2) {
-        std::memcpy(dst, src, n);
-        return;
-    }
-    const size_t per = ((n + parts - 1) / parts + kPart - 1) / kPart * kPart;
-    auto* d = static_cast<char*>(dst);
-    const auto* sp = static_cast<const char*>(src);
-    std::vector<std::thread> th;
-    th.reserve(parts);
-    for (size_t off = per; off < n; off += per) {
-        const size_t len = std::min(per, n - off);
-        try {
-            th.emplace_back([d, sp, off, len] { std::memcpy(d + off, sp + off, len); });
-        } catch (...) {
-            std::memcpy(d + off, sp + off, len);
-        }
-    }
-    std::memcpy(d, sp, std::min(per, n));
-    for (auto& t : th) t.join();
-}
+// Host copy into or out of pinned staging.  One core's memcpy bandwidth (about 8-10 GB/s) is
+// far below the host's, so copies of 4 MB and more are split into 2 MB-aligned parts over a
+// pool of up to 7 worker threads (started on first use, shared by every caller) plus the
+// calling thread.
+void par_memcpy(void* dst, const void* src, size_t n);
 
 struct PinnedBuf {
     void* p = nullptr;

@@ -9,14 +9,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <new>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ghcm_dev.h"
@@ -71,6 +77,101 @@ struct ghcm_ctx {
     PinnedBuf rec;                       // HostRec written by the wave loop
     uint32_t solve_seq = 0;              // tags the totals of each solve
 };
+
+// ---------------------------------------------------------------------------------------
+// parallel host copies
+// ---------------------------------------------------------------------------------------
+
+namespace {
+
+// Worker pool of par_memcpy.  Never destroyed: its threads wait on a condition variable for
+// the life of the process.
+class CopyPool {
+public:
+    static CopyPool& get() {
+        static CopyPool* pool = new CopyPool();
+        return *pool;
+    }
+    size_t workers() const { return threads_.size(); }
+    // Runs parts [1, parts) on the workers and part 0 on the caller; returns when all are done.
+    void run(size_t parts, const std::function<void(size_t)>& part) {
+        struct Batch {
+            std::atomic<size_t> left;
+        } batch{{parts - 1}};
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = 1; i < parts; ++i)
+                queue_.push_back([&part, &batch, i] {
+                    part(i);
+                    batch.left.fetch_sub(1, std::memory_order_release);
+                });
+        }
+        cv_.notify_all();
+        part(0);
+        // Help with queued parts (ours or another caller's) instead of only waiting.
+        while (batch.left.load(std::memory_order_acquire) != 0) {
+            std::function<void()> job;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (!queue_.empty()) {
+                    job = std::move(queue_.front());
+                    queue_.pop_front();
+                }
+            }
+            if (job) job();
+            else std::this_thread::yield();
+        }
+    }
+
+private:
+    CopyPool() {
+        const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+        const size_t n = std::min<size_t>(7, hw > 1 ? hw - 1 : 0);
+        for (size_t i = 0; i < n; ++i) {
+            try {
+                threads_.emplace_back([this] { loop(); });
+                threads_.back().detach();
+            } catch (...) {
+                break;
+            }
+        }
+    }
+    void loop() {
+        for (;;) {
+            std::function<void()> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [this] { return !queue_.empty(); });
+                job = std::move(queue_.front());
+                queue_.pop_front();
+            }
+            job();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::function<void()>> queue_;
+    std::vector<std::thread> threads_;
+};
+
+}  // namespace
+
+void ghcm_capi::par_memcpy(void* dst, const void* src, size_t n) {
+    constexpr size_t kPart = size_t(2) << 20;
+    CopyPool& pool = CopyPool::get();
+    const size_t parts = std::min<size_t>(n / kPart, pool.workers() + 1);
+    if (parts < 2) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = ((n + parts - 1) / parts + kPart - 1) / kPart * kPart;
+    auto* d = static_cast<char*>(dst);
+    const auto* sp = static_cast<const char*>(src);
+    pool.run(parts, [d, sp, n, per](size_t i) {
+        const size_t off = i * per;
+        if (off < n) std::memcpy(d + off, sp + off, std::min(per, n - off));
+    });
+}
 
 // ---------------------------------------------------------------------------------------
 // library / device
