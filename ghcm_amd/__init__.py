@@ -142,6 +142,13 @@ EXPORTED_SYMBOLS = (
 _lib = None
 
 
+class PreparedBatch:
+    """A batch's ghcm_instance array and the seed arrays it points into."""
+
+    def __init__(self, arr, keep, n: int):
+        self.arr, self.keep, self.n = arr, keep, n
+
+
 class GhcmError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"ghcm status {status}: {msg}")
@@ -402,12 +409,20 @@ class Context:
             arr[i] = Instance(df, dt, w, h, s.ctypes.data, s.shape[0], 0)
         return arr, keep
 
-    def solve_batch(self, kind: int, instances: Sequence[tuple]) -> dict:
-        """instances: (d_field_ptr, d_time_ptr, w, h, seeds) tuples, device pointers."""
-        arr, keep = self._instances(instances)
+    def solve_batch(self, kind: int, instances) -> dict:
+        """instances: (d_field_ptr, d_time_ptr, w, h, seeds) tuples (device pointers), or a
+        PreparedBatch from prepare_batch()."""
+        if not isinstance(instances, PreparedBatch):
+            instances = self.prepare_batch(instances)
         st = Stats()
-        _check(self._L.ghcm_solve_batch(self._h, kind, arr, len(instances), ctypes.byref(st)))
+        _check(self._L.ghcm_solve_batch(self._h, kind, instances.arr, instances.n,
+                                        ctypes.byref(st)))
         return st.to_dict()
+
+    def prepare_batch(self, instances: Sequence[tuple]) -> "PreparedBatch":
+        """Build the ghcm_instance array of a batch once (a C caller keeps its array too)."""
+        arr, keep = self._instances(instances)
+        return PreparedBatch(arr, keep, len(instances))
 
     def autotune(self, kind: int, instances: Sequence[tuple], candidates=None, reps: int = 2):
         """Time candidate configurations on the instances; the fastest becomes the config.
