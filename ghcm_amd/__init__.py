@@ -109,10 +109,10 @@ class ServeStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint64) for k in (
         "submitted", "admitted", "completed", "pending", "waves_enqueued", "busy_waves",
         "loop_wave", "peak_live", "budget_bytes", "bytes_in_use", "peak_bytes", "max_tiles")] + \
-        [(k, ctypes.c_uint32) for k in ("slots", "n_active", "waves_per_chunk", "reserved")]
+        [(k, ctypes.c_uint32) for k in ("slots", "n_active", "waves_per_chunk", "waiters")]
 
     def to_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Image(ctypes.Structure):
@@ -507,10 +507,11 @@ class Server:
         """(times (H, W) float32 or None for a device request, result dict); raises
         GhcmError on failure or time-out."""
         r = ServeResult()
+        r.status = -1
         rc = self._L.ghcm_serve_wait(self._h, ticket, timeout_s, ctypes.byref(r))
-        if rc == ERR_TIMEOUT:
-            _check(rc)
-        out = self._held.pop(ticket, None)
+        # The output buffer is released only by the call that collected the ticket: a
+        # refused second waiter must not free it while the collecting call copies into it.
+        out = self._held.pop(ticket, None) if r.status >= 0 else None
         _check(rc)
         return out, r.to_dict()
 

@@ -225,6 +225,7 @@ struct ghcm_server {
     std::unordered_map<uint64_t, Request*> tickets;
     uint64_t next_ticket = 1;
     bool stop = false, drain = true, exited = false;
+    uint32_t waiters = 0;                // ghcm_serve_wait calls still inside (under mu)
     int fatal = GHCM_OK;
     std::string fatal_msg;
     ghcm_serve_stats st{};
@@ -818,6 +819,17 @@ extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s
                                ghcm_serve_result* out) {
     if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
     std::unique_lock<std::mutex> lk(s->mu);
+    // ghcm_serve_stop frees the server only once no wait is inside this function.
+    struct Leave {
+        ghcm_server* s;
+        std::unique_lock<std::mutex>& lk;
+        ~Leave() {
+            if (!lk.owns_lock()) lk.lock();
+            if (--s->waiters == 0) s->cv_done.notify_all();
+        }
+    };
+    ++s->waiters;
+    Leave leave{s, lk};
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end())
         return fail(GHCM_ERR_NOT_FOUND, "ticket %llu unknown or already collected", (unsigned long long)ticket);
@@ -866,6 +878,7 @@ extern "C" int ghcm_serve_get_stats(ghcm_server* s, ghcm_serve_stats* out) {
     *out = s->st;
     out->pending = s->pending.size();
     out->bytes_in_use = s->arena_alloc.used();
+    out->waiters = s->waiters;
     return GHCM_OK;
 }
 
@@ -878,6 +891,11 @@ extern "C" int ghcm_serve_stop(ghcm_server* s, int drain) {
     }
     s->cv_work.notify_all();
     if (s->thread.joinable()) s->thread.join();
+    {
+        // Every request is Done now, so each wait still inside returns promptly.
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [s] { return s->waiters == 0; });
+    }
     (void)hipSetDevice(s->ordinal);
     s->release_all();
     const int fatal = s->fatal;

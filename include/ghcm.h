@@ -268,7 +268,8 @@ typedef struct ghcm_serve_stats {
     uint64_t peak_live;             /* most requests on the device at once            */
     uint64_t budget_bytes, bytes_in_use, peak_bytes;
     uint64_t max_tiles;
-    uint32_t slots, n_active, waves_per_chunk, reserved;
+    uint32_t slots, n_active, waves_per_chunk;
+    uint32_t waiters;               /* ghcm_serve_wait calls in progress              */
 } ghcm_serve_stats;
 
 typedef struct ghcm_server ghcm_server;
@@ -293,11 +294,15 @@ int ghcm_serve_submit_device(ghcm_server* s, const void* d_field, uint32_t w, ui
                              uint64_t* ticket);
 /* Waits up to timeout_s seconds (< 0: no limit).  Returns the request's status (and
  * collects the ticket) or GHCM_ERR_TIMEOUT (the ticket stays valid).  One waiter per
- * ticket.  A host request's times are copied into its `time_out` by this call. */
+ * ticket: a second one gets GHCM_ERR_INVALID_ARG, an unknown or collected ticket
+ * GHCM_ERR_NOT_FOUND.  `out` is written only when this call collects the ticket
+ * (out->status >= 0), and only then may the caller reuse the request's `time_out`:
+ * a host request's times are copied into it by the collecting call. */
 int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s, ghcm_serve_result* out);
 int ghcm_serve_get_stats(ghcm_server* s, ghcm_serve_stats* out);
 /* drain != 0: every submitted request completes first; 0: requests not yet complete
- * end with GHCM_ERR_STOPPED.  Frees the server. */
+ * end with GHCM_ERR_STOPPED.  Waits blocked in ghcm_serve_wait return first; then the
+ * server is freed (no call may use it after this one starts returning). */
 int ghcm_serve_stop(ghcm_server* s, int drain);
 
 /* ---- synthetic fields (seeded, documented generators) ---------------------- */

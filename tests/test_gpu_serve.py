@@ -1,6 +1,7 @@
 """Continuous-batching server (ghcm_serve_*) on the GPU: results alone vs in the mix,
 admission by HBM budget, errors, graph replay, threads.  Needs an MI355X."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -253,3 +254,40 @@ def test_second_waiter_on_same_ticket_is_refused():
     assert len(ok) == 1, got
     other = got["b" if ok[0] == "a" else "a"]
     assert other in (g.ERR_NOT_FOUND, g.ERR_INVALID_ARG)
+    # The collecting waiter gets the caller's buffer, filled; the refused one freed nothing.
+    times = got[ok[0]][0]
+    assert isinstance(times, np.ndarray) and times.shape == (1024, 1024)
+    ref = oracle.fmm_f32(r.cost, r.seeds)
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.isfinite(times), fin)
+    assert np.max(np.abs(times[fin].astype(np.float64) - ref[fin])) <= 1e-4 * float(ref[fin].max())
+
+
+def test_stop_returns_blocked_waiters():
+    """stop(drain=False) while another thread waits with no time limit: the wait returns
+    (its result or GHCM_ERR_STOPPED) before the server is freed."""
+    print("case serve stop with a blocked waiter", flush=True)
+    r = workloads._make(1024, 1024, "ring", 63, 2, 64)
+    srv = g.Server(0, **SWEEP)
+    tickets = [srv.submit(r.cost, r.seeds) for _ in range(4)]
+    got = []
+
+    def waiter(t):
+        try:
+            got.append(srv.wait(t, -1.0)[1]["status"])
+        except g.GhcmError as e:
+            got.append(e.status)
+    ths = [threading.Thread(target=waiter, args=(t,)) for t in tickets]
+    for th in ths:
+        th.start()
+    # Stop only once every thread is inside ghcm_serve_wait or has returned from it: a
+    # thread that had read the handle but not yet called in would use a freed server.
+    deadline = time.monotonic() + 30.0
+    while srv.stats()["waiters"] + len(got) < len(ths):
+        assert time.monotonic() < deadline
+        time.sleep(0.001)
+    srv.stop(drain=False)
+    for th in ths:
+        th.join(30.0)
+        assert not th.is_alive()
+    assert len(got) == 4 and set(got) <= {g.OK, g.ERR_STOPPED, g.ERR_NOT_FOUND}, got
