@@ -63,52 +63,54 @@ extern "C" int ghcm_tune_default_space(int cus, ghcm_tune_candidate* out, uint32
 extern "C" int ghcm_autotune(ghcm_ctx* ctx, int kind, const ghcm_instance* inst, uint32_t n_inst,
                              ghcm_tune_candidate* cands, uint32_t n_cands, uint32_t reps,
                              uint32_t* best) {
-    if (!ctx || !inst || n_inst == 0 || !cands || n_cands == 0)
-        return fail(GHCM_ERR_INVALID_ARG, "autotune: null argument or empty instance/candidate list");
-    ghcm_config base;
-    int rc = ghcm_ctx_get_config(ctx, &base);
-    if (rc) return rc;
-    if (reps == 0) reps = 1;
-    uint32_t best_i = n_cands;
-    for (uint32_t i = 0; i < n_cands; ++i) {
-        ghcm_tune_candidate& c = cands[i];
-        c.device_us = INFINITY;
-        c.converged = 0;
-        c.tile_visits = 0;
-        ghcm_config cfg = base;
-        cfg.tile_w = c.tile_w;
-        cfg.tile_h = c.tile_h;
-        cfg.kernel = c.kernel;
-        cfg.n_active = c.n_active;
-        cfg.inner_iter_cap = c.inner_iter_cap;
-        cfg.launch_mode = c.launch_mode;
-        cfg.profile_kernels = 0;
-        c.status = ghcm_ctx_set_config(ctx, &cfg);
-        if (c.status != GHCM_OK) continue;
-        for (uint32_t r = 0; r < reps; ++r) {
-            ghcm_stats st;
-            c.status = ghcm_solve_batch(ctx, kind, inst, n_inst, &st);
-            if (c.status != GHCM_OK) break;
-            c.converged = st.converged;
-            c.tile_visits = st.tile_visits;
-            if (st.duration_device_us < c.device_us) c.device_us = st.duration_device_us;
+    return ghcm_capi::no_throw([&]() -> int {
+        if (!ctx || !inst || n_inst == 0 || !cands || n_cands == 0)
+            return fail(GHCM_ERR_INVALID_ARG, "autotune: null argument or empty instance/candidate list");
+        ghcm_config base;
+        int rc = ghcm_ctx_get_config(ctx, &base);
+        if (rc) return rc;
+        if (reps == 0) reps = 1;
+        uint32_t best_i = n_cands;
+        for (uint32_t i = 0; i < n_cands; ++i) {
+            ghcm_tune_candidate& c = cands[i];
+            c.device_us = INFINITY;
+            c.converged = 0;
+            c.tile_visits = 0;
+            ghcm_config cfg = base;
+            cfg.tile_w = c.tile_w;
+            cfg.tile_h = c.tile_h;
+            cfg.kernel = c.kernel;
+            cfg.n_active = c.n_active;
+            cfg.inner_iter_cap = c.inner_iter_cap;
+            cfg.launch_mode = c.launch_mode;
+            cfg.profile_kernels = 0;
+            c.status = ghcm_ctx_set_config(ctx, &cfg);
+            if (c.status != GHCM_OK) continue;
+            for (uint32_t r = 0; r < reps; ++r) {
+                ghcm_stats st;
+                c.status = ghcm_solve_batch(ctx, kind, inst, n_inst, &st);
+                if (c.status != GHCM_OK) break;
+                c.converged = st.converged;
+                c.tile_visits = st.tile_visits;
+                if (st.duration_device_us < c.device_us) c.device_us = st.duration_device_us;
+            }
+            if (c.status == GHCM_OK && c.converged &&
+                (best_i == n_cands || c.device_us < cands[best_i].device_us))
+                best_i = i;
         }
-        if (c.status == GHCM_OK && c.converged &&
-            (best_i == n_cands || c.device_us < cands[best_i].device_us))
-            best_i = i;
-    }
-    if (best_i == n_cands) {
-        (void)ghcm_ctx_set_config(ctx, &base);
-        return fail(GHCM_ERR_NOT_CONVERGED, "autotune: no candidate converged");
-    }
-    ghcm_config cfg = base;
-    cfg.tile_w = cands[best_i].tile_w;
-    cfg.tile_h = cands[best_i].tile_h;
-    cfg.kernel = cands[best_i].kernel;
-    cfg.n_active = cands[best_i].n_active;
-    cfg.inner_iter_cap = cands[best_i].inner_iter_cap;
-    cfg.launch_mode = cands[best_i].launch_mode;
-    rc = ghcm_ctx_set_config(ctx, &cfg);
-    if (best) *best = best_i;
-    return rc;
+        if (best_i == n_cands) {
+            (void)ghcm_ctx_set_config(ctx, &base);
+            return fail(GHCM_ERR_NOT_CONVERGED, "autotune: no candidate converged");
+        }
+        ghcm_config cfg = base;
+        cfg.tile_w = cands[best_i].tile_w;
+        cfg.tile_h = cands[best_i].tile_h;
+        cfg.kernel = cands[best_i].kernel;
+        cfg.n_active = cands[best_i].n_active;
+        cfg.inner_iter_cap = cands[best_i].inner_iter_cap;
+        cfg.launch_mode = cands[best_i].launch_mode;
+        rc = ghcm_ctx_set_config(ctx, &cfg);
+        if (best) *best = best_i;
+        return rc;
+    });
 }

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <new>
+#include <stdexcept>
 #include <vector>
 #include <thread>
 
@@ -17,6 +19,21 @@ namespace ghcm_capi {
 
 // Records a printf-style message for ghcm_last_error() and returns `status`.
 int fail(int status, const char* fmt, ...);
+
+// Runs an entry point's body so that no exception crosses the C boundary: a host container
+// that cannot grow becomes GHCM_ERR_OUT_OF_MEMORY.
+template <class F>
+int no_throw(F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(GHCM_ERR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::length_error&) {
+        return fail(GHCM_ERR_OUT_OF_MEMORY, "host allocation too large");
+    } catch (const std::exception& e) {
+        return fail(GHCM_ERR_INVALID_ARG, "%s", e.what());
+    }
+}
 // Validation shared by contexts and servers.
 int check_config(const ghcm_config* c);
 

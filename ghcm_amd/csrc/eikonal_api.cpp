@@ -6,6 +6,7 @@
 // the time field by value with total/device durations.
 #include <iostream>
 
+#include "capi_common.h"
 #include "eikonal.h"
 #include "eikonal_c.h"
 
@@ -132,38 +133,44 @@ int copy_out(const ghcm::EikonalResult& r, float* out, double* us) {
 
 extern "C" int ghcm_cpu_fmm(const float* cost, uint32_t w, uint32_t h, const uint32_t* seeds,
                             uint32_t n, float* out, double* us) {
-    ghcm::Field2D<float> f;
-    std::vector<ghcm::uint2> s;
-    if (!out || !to_inputs(cost, w, h, seeds, n, &f, &s)) return GHCM_ERR_INVALID_ARG;
-    return copy_out(ghcm::eikonal_fmm_sequential(f, s), out, us);
+    return ghcm_capi::no_throw([&]() -> int {
+        ghcm::Field2D<float> f;
+        std::vector<ghcm::uint2> s;
+        if (!out || !to_inputs(cost, w, h, seeds, n, &f, &s)) return GHCM_ERR_INVALID_ARG;
+        return copy_out(ghcm::eikonal_fmm_sequential(f, s), out, us);
+    });
 }
 
 extern "C" int ghcm_cpu_fim(float eps, const float* cost, uint32_t w, uint32_t h,
                             const uint32_t* seeds, uint32_t n, float* out, double* us) {
-    ghcm::Field2D<float> f;
-    std::vector<ghcm::uint2> s;
-    if (!out || !to_inputs(cost, w, h, seeds, n, &f, &s)) return GHCM_ERR_INVALID_ARG;
-    return copy_out(ghcm::eikonal_fim_sequential(eps, f, s), out, us);
+    return ghcm_capi::no_throw([&]() -> int {
+        ghcm::Field2D<float> f;
+        std::vector<ghcm::uint2> s;
+        if (!out || !to_inputs(cost, w, h, seeds, n, &f, &s)) return GHCM_ERR_INVALID_ARG;
+        return copy_out(ghcm::eikonal_fim_sequential(eps, f, s), out, us);
+    });
 }
 
 extern "C" int ghcm_hcm_solve_cxx(int ordinal, uint32_t block_size, float cu_overload, float eps,
                                   int kernel, const float* cost, uint32_t w, uint32_t h,
                                   const uint32_t* seeds, uint32_t n, float* out, ghcm_stats* stats) {
-    ghcm::Field2D<float> f;
-    std::vector<ghcm::uint2> s;
-    if (!out || block_size == 0 || block_size > 255 || !to_inputs(cost, w, h, seeds, n, &f, &s))
-        return GHCM_ERR_INVALID_ARG;
-    ghcm::HcmConfig c;
-    c.block_size = (uint8_t)block_size;
-    c.compute_unit_overload = cu_overload;
-    c.fim_minimum_change = eps;
-    c.kernel = kernel == 1 ? ghcm::HcmConfig::Kernel::FIM
-             : kernel == 2 ? ghcm::HcmConfig::Kernel::Sweep : ghcm::HcmConfig::Kernel::Systolic;
-    auto setup = ghcm::eikonal_hcm_hip_setup(ghcm::HcmDevice{ordinal}, c);
-    if (!setup) return GHCM_ERR_INVALID_ARG;
-    ghcm::EikonalResult r = ghcm::eikonal_hcm_hip_ex(*setup, f, s);
-    if (stats) *stats = r.stats;
-    if (r.status != GHCM_OK) return r.status;
-    std::copy(r.field.backing_store().begin(), r.field.backing_store().end(), out);
-    return GHCM_OK;
+    return ghcm_capi::no_throw([&]() -> int {
+        ghcm::Field2D<float> f;
+        std::vector<ghcm::uint2> s;
+        if (!out || block_size == 0 || block_size > 255 || !to_inputs(cost, w, h, seeds, n, &f, &s))
+            return GHCM_ERR_INVALID_ARG;
+        ghcm::HcmConfig c;
+        c.block_size = (uint8_t)block_size;
+        c.compute_unit_overload = cu_overload;
+        c.fim_minimum_change = eps;
+        c.kernel = kernel == 1 ? ghcm::HcmConfig::Kernel::FIM
+                 : kernel == 2 ? ghcm::HcmConfig::Kernel::Sweep : ghcm::HcmConfig::Kernel::Systolic;
+        auto setup = ghcm::eikonal_hcm_hip_setup(ghcm::HcmDevice{ordinal}, c);
+        if (!setup) return GHCM_ERR_INVALID_ARG;
+        ghcm::EikonalResult r = ghcm::eikonal_hcm_hip_ex(*setup, f, s);
+        if (stats) *stats = r.stats;
+        if (r.status != GHCM_OK) return r.status;
+        std::copy(r.field.backing_store().begin(), r.field.backing_store().end(), out);
+        return GHCM_OK;
+    });
 }

@@ -19,6 +19,7 @@
 //             weights exp(-i^2 / 2 sigma^2) normalised, half-sample 'reflect' boundary),
 //             then log10 speed = log10 lo + (log10 hi - log10 lo)(z - min z)/(max z - min z).
 //   points    x = floor(u W), y = floor(u H), drawn as (x0, y0, x1, y1, ...).
+#include "capi_common.h"
 #include "ghcm.h"
 
 #include <algorithm>
@@ -89,66 +90,70 @@ void gauss_pass(const double* in, double* out, int w, int h, bool along_x,
 
 extern "C" int ghcm_gen_ring_speed(uint32_t w, uint32_t h, uint64_t seed, double wall_speed,
                                    double scale, double* out) {
-    if (!out || w == 0 || h == 0 || !(scale > 0.0) || !(wall_speed >= 0.0)) return GHCM_ERR_INVALID_ARG;
-    SplitMix64 rng(seed);
-    const double two_pi = 6.283185307179586476925;
-    double theta[7], radius[7];
-    for (int k = 0; k < 7; ++k) {
-        theta[k] = two_pi * rng.uniform();
-        radius[k] = scale * (16.0 + k * (120.0 - 16.0) / 6.0);
-    }
-    const double cx = (double)(w / 2), cy = (double)(h / 2);
-    const double half_thick = 1.5 * scale;
-    for (uint32_t y = 0; y < h; ++y) {
-        for (uint32_t x = 0; x < w; ++x) {
-            const double dx = (double)x - cx, dy = (double)y - cy;
-            const double r = std::sqrt(dx * dx + dy * dy);
-            double v = 1.0;
-            for (int k = 0; k < 7; ++k) {
-                if (std::fabs(r - radius[k]) < half_thick) {
-                    double ang = std::atan2(dy, dx);
-                    if (ang < 0) ang += two_pi;
-                    double da = std::fabs(ang - theta[k]);
-                    da = std::min(da, two_pi - da);
-                    if (!(da < 3.0 * scale / radius[k])) v = wall_speed;
-                    break;
-                }
-            }
-            out[(size_t)y * w + x] = v;
+    return ghcm_capi::no_throw([&]() -> int {
+        if (!out || w == 0 || h == 0 || !(scale > 0.0) || !(wall_speed >= 0.0)) return GHCM_ERR_INVALID_ARG;
+        SplitMix64 rng(seed);
+        const double two_pi = 6.283185307179586476925;
+        double theta[7], radius[7];
+        for (int k = 0; k < 7; ++k) {
+            theta[k] = two_pi * rng.uniform();
+            radius[k] = scale * (16.0 + k * (120.0 - 16.0) / 6.0);
         }
-    }
-    return GHCM_OK;
+        const double cx = (double)(w / 2), cy = (double)(h / 2);
+        const double half_thick = 1.5 * scale;
+        for (uint32_t y = 0; y < h; ++y) {
+            for (uint32_t x = 0; x < w; ++x) {
+                const double dx = (double)x - cx, dy = (double)y - cy;
+                const double r = std::sqrt(dx * dx + dy * dy);
+                double v = 1.0;
+                for (int k = 0; k < 7; ++k) {
+                    if (std::fabs(r - radius[k]) < half_thick) {
+                        double ang = std::atan2(dy, dx);
+                        if (ang < 0) ang += two_pi;
+                        double da = std::fabs(ang - theta[k]);
+                        da = std::min(da, two_pi - da);
+                        if (!(da < 3.0 * scale / radius[k])) v = wall_speed;
+                        break;
+                    }
+                }
+                out[(size_t)y * w + x] = v;
+            }
+        }
+        return GHCM_OK;
+    });
 }
 
 extern "C" int ghcm_gen_smooth_speed(uint32_t w, uint32_t h, uint64_t seed, double sigma,
                                      double lo, double hi, double* out) {
-    if (!out || w == 0 || h == 0 || !(sigma > 0.0) || !(lo > 0.0) || !(hi >= lo)) return GHCM_ERR_INVALID_ARG;
-    const size_t n = (size_t)w * h;
-    std::vector<double> z(n), tmp(n);
-    SplitMix64 rng(seed);
-    const double two_pi = 6.283185307179586476925;
-    for (size_t i = 0; i < n; i += 2) {
-        double u1 = rng.uniform(), u2 = rng.uniform();
-        double r = std::sqrt(-2.0 * std::log(1.0 - u1));
-        z[i] = r * std::cos(two_pi * u2);
-        if (i + 1 < n) z[i + 1] = r * std::sin(two_pi * u2);
-    }
-    const int radius = (int)(4.0 * sigma + 0.5);
-    std::vector<double> wt(2 * radius + 1);
-    double sum = 0.0;
-    for (int k = -radius; k <= radius; ++k) {
-        wt[k + radius] = std::exp(-0.5 * (double)k * k / (sigma * sigma));
-        sum += wt[k + radius];
-    }
-    for (double& v : wt) v /= sum;
-    gauss_pass(z.data(), tmp.data(), (int)w, (int)h, true, wt, radius);
-    gauss_pass(tmp.data(), z.data(), (int)w, (int)h, false, wt, radius);
-    double zmin = z[0], zmax = z[0];
-    for (double v : z) { zmin = std::min(zmin, v); zmax = std::max(zmax, v); }
-    const double llo = std::log10(lo), lhi = std::log10(hi);
-    const double span = zmax > zmin ? zmax - zmin : 1.0;
-    for (size_t i = 0; i < n; ++i) out[i] = std::pow(10.0, llo + (lhi - llo) * (z[i] - zmin) / span);
-    return GHCM_OK;
+    return ghcm_capi::no_throw([&]() -> int {
+        if (!out || w == 0 || h == 0 || !(sigma > 0.0) || !(lo > 0.0) || !(hi >= lo)) return GHCM_ERR_INVALID_ARG;
+        const size_t n = (size_t)w * h;
+        std::vector<double> z(n), tmp(n);
+        SplitMix64 rng(seed);
+        const double two_pi = 6.283185307179586476925;
+        for (size_t i = 0; i < n; i += 2) {
+            double u1 = rng.uniform(), u2 = rng.uniform();
+            double r = std::sqrt(-2.0 * std::log(1.0 - u1));
+            z[i] = r * std::cos(two_pi * u2);
+            if (i + 1 < n) z[i + 1] = r * std::sin(two_pi * u2);
+        }
+        const int radius = (int)(4.0 * sigma + 0.5);
+        std::vector<double> wt(2 * radius + 1);
+        double sum = 0.0;
+        for (int k = -radius; k <= radius; ++k) {
+            wt[k + radius] = std::exp(-0.5 * (double)k * k / (sigma * sigma));
+            sum += wt[k + radius];
+        }
+        for (double& v : wt) v /= sum;
+        gauss_pass(z.data(), tmp.data(), (int)w, (int)h, true, wt, radius);
+        gauss_pass(tmp.data(), z.data(), (int)w, (int)h, false, wt, radius);
+        double zmin = z[0], zmax = z[0];
+        for (double v : z) { zmin = std::min(zmin, v); zmax = std::max(zmax, v); }
+        const double llo = std::log10(lo), lhi = std::log10(hi);
+        const double span = zmax > zmin ? zmax - zmin : 1.0;
+        for (size_t i = 0; i < n; ++i) out[i] = std::pow(10.0, llo + (lhi - llo) * (z[i] - zmin) / span);
+        return GHCM_OK;
+    });
 }
 
 extern "C" int ghcm_gen_points(uint32_t w, uint32_t h, uint64_t seed, uint32_t n, uint32_t* xy) {

@@ -39,6 +39,7 @@ using ghcm_capi::bytes_per_visit;
 using ghcm_capi::bytes_per_visit3d;
 using ghcm_capi::fail;
 using ghcm_capi::kind_bytes;
+using ghcm_capi::no_throw;
 using ghcm_capi::valid_kind;
 
 namespace {
@@ -365,8 +366,16 @@ struct HostInst {      // host view of one instance
 // One device wave loop over n_inst instances.  three_d: every instance is a W x H x D grid
 // solved in 8x8x8 tiles by the 3-D tile kernel (device scheduler, per-wave launches or
 // graph replay); the configuration's tile shape and in-tile style do not apply.
+int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
+                    bool three_d);
+
 int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
                bool three_d = false) {
+    return no_throw([&] { return solve_core_body(ctx, kind, hi, n_inst, stats, three_d); });
+}
+
+int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
+                    bool three_d) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     const ghcm_config& cfg = ctx->cfg;
     if (!valid_kind(kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", kind);
@@ -964,10 +973,12 @@ extern "C" int ghcm_rgb_luminosity_device(ghcm_ctx* ctx, const float* d_rgb, uin
 
 extern "C" int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instance* inst,
                                 uint32_t n_inst, ghcm_stats* stats) {
-    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
-    if (!inst || n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
-    std::vector<HostInst> hi(n_inst);
-    for (uint32_t i = 0; i < n_inst; ++i)
-        hi[i] = HostInst{inst[i].d_field, inst[i].d_time, inst[i].w, inst[i].h, inst[i].seeds_xy, inst[i].n_seeds};
-    return solve_core(ctx, field_kind, hi.data(), n_inst, stats);
+    return no_throw([&]() -> int {
+        if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+        if (!inst || n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
+        std::vector<HostInst> hi(n_inst);
+        for (uint32_t i = 0; i < n_inst; ++i)
+            hi[i] = HostInst{inst[i].d_field, inst[i].d_time, inst[i].w, inst[i].h, inst[i].seeds_xy, inst[i].n_seeds};
+        return solve_core(ctx, field_kind, hi.data(), n_inst, stats);
+    });
 }

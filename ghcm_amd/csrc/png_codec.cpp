@@ -146,7 +146,9 @@ const uint16_t kDistBase[30] = {1, 2, 3, 4, 5, 7, 9, 13, 17, 25, 33, 49, 65, 97,
 const uint8_t kDistExtra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6, 6,
                                 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
 
-bool inflate_raw(BitReader& br, std::vector<uint8_t>& out, std::string& err) {
+// Inflation stops once `limit` bytes are out (what follows is not needed), so a stream never
+// grows the output past what the caller can use.
+bool inflate_raw(BitReader& br, std::vector<uint8_t>& out, size_t limit, std::string& err) {
     Huffman lit, dist;
     for (;;) {
         const uint32_t final_block = br.bits(1);
@@ -155,7 +157,9 @@ bool inflate_raw(BitReader& br, std::vector<uint8_t>& out, std::string& err) {
             br.align_byte();
             const uint32_t len = br.bits(16), nlen = br.bits(16);
             if ((len ^ 0xFFFFu) != nlen) { err = "stored block length mismatch"; return false; }
-            for (uint32_t i = 0; i < len; ++i) out.push_back((uint8_t)br.bits(8));
+            const uint32_t take = (uint32_t)std::min<size_t>(len, limit - out.size());
+            for (uint32_t i = 0; i < take; ++i) out.push_back((uint8_t)br.bits(8));
+            if (take < len) return true;
         } else if (type == 1 || type == 2) {
             uint8_t lens[320];
             int hlit = 288, hdist = 32;
@@ -205,6 +209,7 @@ bool inflate_raw(BitReader& br, std::vector<uint8_t>& out, std::string& err) {
                 const int sym = lit.decode(br);
                 if (sym < 0) { err = "bad literal/length code"; return false; }
                 if (sym < 256) {
+                    if (out.size() >= limit) return true;
                     out.push_back((uint8_t)sym);
                 } else if (sym == 256) {
                     break;
@@ -217,7 +222,9 @@ bool inflate_raw(BitReader& br, std::vector<uint8_t>& out, std::string& err) {
                     const uint32_t d = kDistBase[ds] + br.bits(kDistExtra[ds]);
                     if (d > out.size()) { err = "distance beyond output"; return false; }
                     const size_t from = out.size() - d;
-                    for (uint32_t k = 0; k < len; ++k) out.push_back(out[from + k]);
+                    const uint32_t take = (uint32_t)std::min<size_t>(len, limit - out.size());
+                    for (uint32_t k = 0; k < take; ++k) out.push_back(out[from + k]);
+                    if (take < len) return true;
                 }
                 if (br.overrun()) { err = "truncated deflate stream"; return false; }
             }
@@ -292,7 +299,8 @@ void row_samples(const uint8_t* row, uint32_t w, int channels, int depth, uint16
 
 uint32_t png_crc32(const uint8_t* p, size_t n) { return crc32_update(0, p, n); }
 
-bool zlib_inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out, std::string& err) {
+bool zlib_inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out, std::string& err,
+                  size_t limit) {
     if (n < 2) { err = "zlib stream too short"; return false; }
     const int cmf = src[0], flg = src[1];
     if ((cmf & 15) != 8 || ((cmf << 8) | flg) % 31 != 0 || (flg & 0x20)) {
@@ -300,7 +308,7 @@ bool zlib_inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out, std::
         return false;
     }
     BitReader br{src + 2, src + n};
-    if (!inflate_raw(br, out, err)) return false;
+    if (!inflate_raw(br, out, limit, err)) return false;
     return true;
 }
 
@@ -360,17 +368,29 @@ bool png_decode(const uint8_t* data, size_t n, PngImage& img, std::string& err) 
     if (interlace > 1) { err = "bad interlace method"; return false; }
     if (ctype == 3 && (plte.empty() || plte.size() % 3)) { err = "missing or bad palette"; return false; }
 
-    std::vector<uint8_t> raw;
-    if (!zlib_inflate(idat.data(), idat.size(), raw, err)) return false;
-
     const int bits_pp = channels * depth;
+    static const int x0[7] = {0, 4, 0, 2, 0, 1, 0}, y0[7] = {0, 0, 4, 0, 2, 0, 1};
+    static const int dx[7] = {8, 8, 4, 4, 2, 2, 1}, dy[7] = {8, 8, 8, 4, 4, 2, 2};
+    const int passes = interlace ? 7 : 1;
+    // Filtered bytes the header implies (each row: a filter byte and its packed samples).
+    // The inflated stream may not exceed it, and must reach it before anything is sized by
+    // the header: a tiny IDAT under a huge IHDR allocates nothing large.
+    size_t total = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int px0 = interlace ? x0[pass] : 0, py0 = interlace ? y0[pass] : 0;
+        const int pdx = interlace ? dx[pass] : 1, pdy = interlace ? dy[pass] : 1;
+        const size_t pw = w > (uint32_t)px0 ? (w - px0 + pdx - 1) / pdx : 0;
+        const size_t ph = h > (uint32_t)py0 ? (h - py0 + pdy - 1) / pdy : 0;
+        if (pw && ph) total += ((pw * bits_pp + 7) / 8 + 1) * ph;
+    }
+    std::vector<uint8_t> raw;
+    if (!zlib_inflate(idat.data(), idat.size(), raw, err, total)) return false;
+    if (raw.size() < total) { err = "image data too short"; return false; }
+
     const int bpp = std::max(1, bits_pp / 8);
     // Samples of the whole image, 16 bits each, channels interleaved.
     std::vector<uint16_t> smp((size_t)w * h * channels);
     size_t off = 0;
-    const int passes = interlace ? 7 : 1;
-    static const int x0[7] = {0, 4, 0, 2, 0, 1, 0}, y0[7] = {0, 0, 4, 0, 2, 0, 1};
-    static const int dx[7] = {8, 8, 4, 4, 2, 2, 1}, dy[7] = {8, 8, 8, 4, 4, 2, 2};
     std::vector<uint8_t> rows;
     std::vector<uint16_t> line;
     for (int pass = 0; pass < passes; ++pass) {

@@ -17,7 +17,9 @@ struct PngImage {
     std::vector<uint16_t> samples;
 };
 
-bool zlib_inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out, std::string& err);
+// Inflates a zlib stream, stopping once `limit` bytes are out.
+bool zlib_inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out, std::string& err,
+                  size_t limit = SIZE_MAX);
 std::vector<uint8_t> zlib_deflate(const uint8_t* src, size_t n);
 uint32_t png_crc32(const uint8_t* p, size_t n);
 

@@ -742,7 +742,11 @@ int check_request(ghcm_server* s, uint32_t w, uint32_t h, const uint32_t* seeds_
     r->h = h;
     r->tiles_x = tx;
     r->tiles_y = ty;
-    r->seeds.assign(seeds_xy, seeds_xy + 2ull * n_seeds);
+    try {
+        r->seeds.assign(seeds_xy, seeds_xy + 2ull * n_seeds);
+    } catch (const std::bad_alloc&) {
+        return fail(GHCM_ERR_OUT_OF_MEMORY, "%u seeds", n_seeds);
+    }
     return GHCM_OK;
 }
 
@@ -756,9 +760,18 @@ int enqueue_request(ghcm_server* s, Request* r, uint64_t* ticket) {
             return s->fatal ? fail(s->fatal, "server failed: %s", s->fatal_msg.c_str())
                             : fail(GHCM_ERR_STOPPED, "server is stopping");
         }
-        r->ticket = s->next_ticket++;
-        s->tickets[r->ticket] = r;
-        s->pending.push_back(r);
+        r->ticket = s->next_ticket;
+        try {
+            s->tickets[r->ticket] = r;
+            s->pending.push_back(r);
+        } catch (const std::bad_alloc&) {
+            s->tickets.erase(r->ticket);
+            s->pool.put(r->h_field, (size_t)r->w * r->h * kind_bytes(s->kind));
+            s->pool.put(r->h_time, (size_t)r->w * r->h * 4);
+            delete r;
+            return fail(GHCM_ERR_OUT_OF_MEMORY, "request queue allocation failed");
+        }
+        ++s->next_ticket;
         ++s->st.submitted;
     }
     s->cv_work.notify_one();
@@ -773,6 +786,8 @@ extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, 
                                  uint64_t* ticket) {
     if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
     if (!field || !time_out || !ticket) return fail(GHCM_ERR_INVALID_ARG, "null argument");
+    // The pinned staging below is mapped for, and addressed on, the server's device.
+    HIP_TRY(hipSetDevice(s->ordinal));
     Request* r = new (std::nothrow) Request();
     if (!r) return fail(GHCM_ERR_OUT_OF_MEMORY, "request allocation failed");
     int rc = check_request(s, w, h, seeds_xy, n_seeds, false, r);

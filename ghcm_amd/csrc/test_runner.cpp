@@ -212,41 +212,43 @@ extern "C" int ghcm_test_run(const char* solver, ghcm_ctx* ctx, const char* solv
                              const char* job_name, const float* cost, uint32_t w, uint32_t h,
                              const uint32_t* seeds_xy, uint32_t n_seeds, const char* out_dir,
                              uint32_t trials, float fim_minimum_change) {
-    if (!solver || !solver_name || !job_name || !cost || !out_dir || !w || !h || (n_seeds && !seeds_xy))
-        return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "null or empty argument");
-    const std::string kind = solver;
-    ghcm::SolverFn fn;
-    int status = GHCM_OK;
-    if (kind == "fmm") {
-        fn = ghcm::eikonal_fmm_sequential;
-    } else if (kind == "fim") {
-        fn = [fim_minimum_change](ghcm::Field2D<float> const& f, std::vector<ghcm::uint2> const& s) {
-            return ghcm::eikonal_fim_sequential(fim_minimum_change, f, s);
-        };
-    } else if (kind == "hcm") {
-        if (!ctx) return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "the hcm solver needs a context");
-        fn = [ctx, &status](ghcm::Field2D<float> const& f, std::vector<ghcm::uint2> const& s) {
-            ghcm::EikonalResult r;
-            std::vector<uint32_t> xy;
-            for (auto&& p : s) { xy.push_back(p.x); xy.push_back(p.y); }
-            r.field = ghcm::Field2D<float>(f.size());
-            const int rc = ghcm_solve(ctx, f.backing_store().data(), GHCM_FIELD_COST_F32, f.size().x,
-                                      f.size().y, xy.data(), (uint32_t)s.size(),
-                                      r.field.backing_store().data(), &r.stats);
-            if (rc != GHCM_OK) status = rc;
-            r.status = rc;
-            r.duration_total = ghcm::EikonalResult::duration_t((long long)r.stats.duration_total_us);
-            r.duration_device = ghcm::EikonalResult::duration_t((long long)r.stats.duration_device_us);
-            return r;
-        };
-    } else {
-        return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "unknown solver %s (fmm, fim or hcm)", solver);
-    }
-    ghcm::Field2D<float> field(ghcm::uint2(w, h), cost);
-    std::vector<ghcm::uint2> seeds;
-    for (uint32_t i = 0; i < n_seeds; ++i) seeds.emplace_back(seeds_xy[2 * i], seeds_xy[2 * i + 1]);
-    ghcm::test_run(solver_name, {ghcm::Job{job_name, std::move(field)}}, seeds, fn, out_dir, trials);
-    return status;
+    return ghcm_capi::no_throw([&]() -> int {
+        if (!solver || !solver_name || !job_name || !cost || !out_dir || !w || !h || (n_seeds && !seeds_xy))
+            return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "null or empty argument");
+        const std::string kind = solver;
+        ghcm::SolverFn fn;
+        int status = GHCM_OK;
+        if (kind == "fmm") {
+            fn = ghcm::eikonal_fmm_sequential;
+        } else if (kind == "fim") {
+            fn = [fim_minimum_change](ghcm::Field2D<float> const& f, std::vector<ghcm::uint2> const& s) {
+                return ghcm::eikonal_fim_sequential(fim_minimum_change, f, s);
+            };
+        } else if (kind == "hcm") {
+            if (!ctx) return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "the hcm solver needs a context");
+            fn = [ctx, &status](ghcm::Field2D<float> const& f, std::vector<ghcm::uint2> const& s) {
+                ghcm::EikonalResult r;
+                std::vector<uint32_t> xy;
+                for (auto&& p : s) { xy.push_back(p.x); xy.push_back(p.y); }
+                r.field = ghcm::Field2D<float>(f.size());
+                const int rc = ghcm_solve(ctx, f.backing_store().data(), GHCM_FIELD_COST_F32, f.size().x,
+                                          f.size().y, xy.data(), (uint32_t)s.size(),
+                                          r.field.backing_store().data(), &r.stats);
+                if (rc != GHCM_OK) status = rc;
+                r.status = rc;
+                r.duration_total = ghcm::EikonalResult::duration_t((long long)r.stats.duration_total_us);
+                r.duration_device = ghcm::EikonalResult::duration_t((long long)r.stats.duration_device_us);
+                return r;
+            };
+        } else {
+            return ghcm_capi::fail(GHCM_ERR_INVALID_ARG, "unknown solver %s (fmm, fim or hcm)", solver);
+        }
+        ghcm::Field2D<float> field(ghcm::uint2(w, h), cost);
+        std::vector<ghcm::uint2> seeds;
+        for (uint32_t i = 0; i < n_seeds; ++i) seeds.emplace_back(seeds_xy[2 * i], seeds_xy[2 * i + 1]);
+        ghcm::test_run(solver_name, {ghcm::Job{job_name, std::move(field)}}, seeds, fn, out_dir, trials);
+        return status;
+    });
 }
 
 extern "C" int ghcm_field_cost_sin_checkerboard(uint32_t w, uint32_t h, float amp, float res, float* out) {

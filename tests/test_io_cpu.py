@@ -285,6 +285,41 @@ def test_blob_layout(tmp_path):
     assert np.array_equal(g.blob_read(p), f)
 
 
+def test_blob_with_a_wrapping_size_header_is_refused(tmp_path):
+    """2^31 x 2^31 floats: x * y * 4 wraps to 0 in 64 bits, which an empty payload would
+    match; the header must not be believed."""
+    p = tmp_path / "bad.blob"
+    x = y = 1 << 31
+    assert (x * y * 4) % (1 << 64) == 0
+    p.write_bytes(struct.pack("=QQ", x, y))
+    with pytest.raises(g.GhcmError):
+        g.blob_read(p)
+
+
+def _ihdr_png(w, h, idat: bytes) -> bytes:
+    return (b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 0, 0, 0, 0))
+            + _chunk(b"IDAT", idat) + _chunk(b"IEND", b""))
+
+
+def test_png_with_huge_header_and_tiny_data_is_refused():
+    """A 16M x 16M grey header over a few bytes of data: an error, not an allocation."""
+    png = _ihdr_png(1 << 24, 1 << 24, zlib.compress(bytes(64)))
+    with pytest.raises(g.GhcmError, match="too short"):
+        g.png_decode8(png)
+
+
+def test_png_inflation_stops_at_the_image_size():
+    """A stream that inflates far past what the header needs (a small zlib bomb) decodes
+    the image it describes and ignores the rest, as before the bound."""
+    w, h = 16, 4
+    rows = b"".join(b"\x00" + bytes(range(y, y + w)) for y in range(h))
+    png = _ihdr_png(w, h, zlib.compress(rows + bytes(50 << 20), 9))
+    assert len(png) < 200_000
+    out = g.png_decode8(png)
+    assert out.shape == (h, w, 1)
+    assert np.array_equal(out[:, :, 0], np.array([list(range(y, y + w)) for y in range(h)], np.uint8))
+
+
 def test_test_run_outputs_and_skip_rule(tmp_path):
     speed = g.gen_ring_speed(64, 48, 3, 1e-3, 0.25)
     cost = g.cost_from_speed(speed)
