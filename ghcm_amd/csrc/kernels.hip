@@ -80,11 +80,14 @@ __device__ __forceinline__ float load_cost_at(const char* p) {
 // lower each other by an ulp at a time without end.)
 // Every update of a cell and every border prediction of a neighbour's update goes through
 // this one function.
+// (d + root)/2 is formed as fma(root, 1/2, d/2): halving is exact for normal numbers, so the
+// fma's one rounding of root/2 + d/2 equals half the rounded d + root, bit for bit, and d/2
+// is ready before the root, which takes one dependent operation off each step's chain.
 __device__ __forceinline__ float godunov_raw(float ax, float ay, float m, float c) {
     const float d = fabsf(ax - ay);                // NaN when both are +INF
     const float s = 1.41421356237309515f * c;
     // (s - d)(s + d) >= 0 whenever c >= d; the other branch discards it.
-    const float two = 0.5f * (d + __builtin_amdgcn_sqrtf((s - d) * (s + d)));
+    const float two = __builtin_fmaf(__builtin_amdgcn_sqrtf((s - d) * (s + d)), 0.5f, 0.5f * d);
     return m + ((c >= d) ? two : c);
 }
 
@@ -636,7 +639,6 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     // one step after lane L-1, so the ordered passes of the lanes form one sweep front.
     const int p0 = (-L) & (TH - 1);
     const int r_start = down ? p0 : TH - 1 - p0;
-    int r = r_start;
     uint32_t rounds = 0;
     bool unfinished = false;
     // LDS reads of step s+1 are issued before step s is computed (two register sets): their
@@ -648,6 +650,17 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     // and the store are immediate offsets from one address.
     char* const shb = reinterpret_cast<char*>(sh);
     const int xc4 = 4 * xc;
+    // Lane L is at the first row of its pass at the steps s with s = L (mod TH): a scalar
+    // one-hot pattern shifted by s & (TH-1) marks those lanes, and the row's byte offset
+    // advances by one row per step, back to the first row exactly there (the wrap of the row
+    // index coincides with the pass start).  This keeps the row bookkeeping out of the
+    // vector ALU: two operations per step instead of four.
+    constexpr uint64_t kFirstPat = TH >= 64 ? 1ull : ~0ull / ((1ull << (TH & 63)) - 1ull);
+    const int b_step = step_dir * (4 * PW);
+    const int b_first = r_first * (4 * PW) + xc4;
+    const int b_start = r_start * (4 * PW) + xc4;
+    int bb = b_start - b_step;
+    uint32_t jl = 0;
     struct Nb {
         float tc, cc, tl, tr, tu, td;
         int b;
@@ -655,15 +668,16 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     };
     auto at = [&](int b, int word) -> float { return *reinterpret_cast<const float*>(shb + b + 4 * word); };
     auto load = [&](Nb& o) {
-        o.b = r * (4 * PW) + xc4;
-        o.first = r == r_first;
+        o.first = __builtin_amdgcn_inverse_ballot_w64(kFirstPat << (jl & (TH - 1)));
+        ++jl;
+        bb = o.first ? b_first : bb + b_step;
+        o.b = bb;
         o.tu = at(o.b, 1);
         o.tl = at(o.b, PW);
         o.tc = at(o.b, PW + 1);
         o.tr = at(o.b, PW + 2);
         o.td = at(o.b, 2 * PW + 1);
         o.cc = at(o.b, CB + PW + 1);
-        r = (r + step_dir) & (TH - 1);
     };
     // Destination of the DPP move, carried from step to step: lane 0 has no source lane, is
     // never written and keeps +INF (its left neighbour is the halo, read from LDS as n.tl),
@@ -672,7 +686,8 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     for (;;) {
         float prev = __builtin_inff();
         float maxdec = 0.0f;
-        r = r_start;
+        bb = b_start - b_step;
+        jl = 0;
         // Times are non-negative or +INF, so their bit patterns order like the values: the
         // axis minima and the merge use integer minimum (no NaN canonicalisation needed).
         auto umin = [](float u, float v) {
