@@ -935,8 +935,14 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
     const uint32_t cur0 = ctrl->cur;
     const uint32_t wave0 = ctrl->wave;
     const uint32_t n = ctrl->n_select;
+    // Read here, with the other words, and written back below without a second round trip:
+    // nothing else writes it while a boundary runs.
+    const uint32_t busy0 = ctrl->busy_waves;
     uint32_t cnt = 0u;
     if (tid < 2 * (int)kShards) cnt = ctrl->count[tid >> 5][(tid & 31) * kCounterStride];
+    // Keeps the compiler from sinking these reads (and the `first` argument) below the done
+    // test, which would cost a second round trip on every boundary.  The statement is empty.
+    asm volatile("" ::"s"(busy0), "s"((uint32_t)first));
     if (done) return -1;
     // list[cur0] was consumed by the wave that just ran (unless `first`): it is emptied and
     // the other list becomes current.
@@ -966,7 +972,7 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
         ctrl->wave = wave;
         ctrl->total = c;
         if (c == 0u) ctrl->done = 1u;
-        else ctrl->busy_waves += 1u;
+        else ctrl->busy_waves = busy0 + 1u;
         ctrl->jobs_from_list = 1u;
         ctrl->n_jobs = c;
     }
@@ -1112,10 +1118,12 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     __shared__ uint32_t sg[kSelectStage];
     __shared__ uint32_t sk[kSelectStage];
     const int tid = threadIdx.x;
+    // Read together with the control words (one round trip); only this kernel writes it.
+    const uint32_t ticks0 = a.ctrl->ticks;
     const int cur = wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc,
                                                     sg, sk, true);
     // An idle launch (the loop is done) only advances the tick, so the host still sees progress.
-    if (tid == 0) a.ctrl->ticks += 1u;
+    if (tid == 0) a.ctrl->ticks = ticks0 + 1u;
     if (cur < 0 || !a.rec || !a.n_slots) return;
     __syncthreads();
     publish_slots<1024>(a, s_prefix, (uint32_t)cur, hist);
