@@ -858,23 +858,32 @@ void k_solve_sweep(SolveArgs a) {
         if (j >= a.n_host_jobs) return;
         sweep_tile<TW, TH, KIND, true>(a, a.host_jobs[j].id, 0u, 0u, 0u, j, sh, s_u32, s_f32, in0);
     } else {
-        __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
         publish_progress(a, ctrl);
         // The control words and the shard prefix are read in one round trip; the done check
-        // follows.
+        // follows.  Every wave keeps the prefix in its lanes 0..kShards-1.
+        const int lane = tid & 63;
         const uint32_t done = ctrl->done;
         const uint32_t nj = ctrl->n_jobs;
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
-        const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
+        // Every lane reads a valid word (no branch around the load).
+        const uint32_t pr = ctrl->prefix[lane & (int)(kShards - 1)];
+        const uint32_t pv = lane < (int)kShards ? pr : 0xFFFFFFFFu;
         if (done || blockIdx.x >= nj) return;
         const uint32_t p = wave & 1u;
-        if (tid <= (int)kShards) s_prefix[tid] = pv;
-        __syncthreads();
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
-            const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+            uint32_t g;
+            if (from_list) {
+                // Entry j of list[cur]: its shard is the last one whose prefix is <= j (the
+                // prefix is non-decreasing and starts at 0), found with one ballot.
+                const uint32_t sh = (uint32_t)__popcll(__ballot(pv <= j)) - 1u;
+                const uint32_t base = __builtin_amdgcn_readlane(pv, (int)sh);
+                g = a.list[((size_t)cur * kShards + sh) * a.shard_cap + (j - base)];
+            } else {
+                g = a.jobs[j];
+            }
             if (tid == 0) claim_tile(a, g, p);
             sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
