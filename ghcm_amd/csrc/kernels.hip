@@ -1596,13 +1596,15 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
     }
     __syncthreads();
     int q[CPT];
-    float told[CPT];
+    float told[CPT], tcur[CPT], cost[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int i = tid + kT3Threads * k;
         const int x = i % B, y = (i / B) % B, z = i / (B * B);
         q[k] = ((z + 1) * PB + (y + 1)) * PB + (x + 1);
         told[k] = t_s[q[k]];
+        tcur[k] = told[k];
+        cost[k] = c_s[q[k]];
     }
 
     // In-tile passes.  Out-of-grid cells hold +INF time and cost and stay +INF.
@@ -1612,7 +1614,9 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
     bool unfinished = false;
     // Passes need no barrier between them (updates only lower cells, so a racing read sees an
     // upper bound); the block-wide convergence vote is taken every kT3PassesPerVote passes,
-    // which cut the 256^3 volume solve from 5.08 to 4.82 ms (every 4: 4.87 ms).
+    // which cut the 256^3 volume solve from 5.08 to 4.82 ms (every 4: 4.87 ms).  A thread is
+    // the only writer of its own cells during the passes, so their times (and costs) stay in
+    // registers and only the six neighbours are read from LDS.
     for (;;) {
         bool changed = false;
 #pragma unroll
@@ -1620,13 +1624,14 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
                 const int c = q[k];
-                const float t0 = t_s[c];
+                const float t0 = tcur[k];
                 const float ax = fminf(t_s[c - 1], t_s[c + 1]);
                 const float ay = fminf(t_s[c - PB], t_s[c + PB]);
                 const float az = fminf(t_s[c - PB2], t_s[c + PB2]);
-                const float tn = godunov3_raw(ax, ay, az, c_s[c]);
+                const float tn = godunov3_raw(ax, ay, az, cost[k]);
                 if (tn < t0) {
                     t_s[c] = tn;
+                    tcur[k] = tn;
                     if (t0 - tn >= eps) changed = true;
                 }
             }
@@ -1720,7 +1725,7 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
 }
 
 template <int KIND>
-__global__ __launch_bounds__(kT3Threads) void k_solve3d(SolveArgs a) {
+__global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8))) void k_solve3d(SolveArgs a) {
     __shared__ float t_s[kT3Box * kT3Box * kT3Box];
     __shared__ float c_s[kT3Box * kT3Box * kT3Box];
     __shared__ uint32_t s_u32[16];
