@@ -1037,54 +1037,106 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
             __syncthreads();
         }
         // Compaction: keys < lo_base (there are `below` of them) all go; keys in
-        // [lo_base, lo_base + 128) go while the quota n - below lasts.  The histogram is free
-        // from here on; its first kShards words count the carried entries per shard.
+        // [lo_base, lo_base + 128) go, in list order, while the quota n - below lasts.  Jobs
+        // keep list order.  The histogram is free from here on; its first kShards words count
+        // the carried entries per shard.
         if (tid < (int)kShards) hist[tid] = 0u;
         __syncthreads();
         const uint32_t quota = n - below;
-        uint32_t quota_used = 0;
         const uint32_t thr_hi = lo_base + (1u << 7);
-        for (uint32_t i0 = 0; i0 < c; i0 += NT) {
-            const uint32_t i = i0 + tid;
-            uint32_t g = 0, k = 0xFFFFFFFFu, low = 0, tie = 0;
-            if (i < c) {
-                g = entry_g(i);
-                k = entry_k(i, g);
-                low = k < lo_base ? 1u : 0u;
-                tie = (!low && k < thr_hi) ? 1u : 0u;
-            }
-            uint32_t tot_tie;
-            const uint32_t tie_rank = block_excl_scan<NT>(tie, part, &tot_tie) + quota_used;
-            const uint32_t sel = low | ((tie && tie_rank < quota) ? 1u : 0u);
-            uint32_t tot_sel;
-            const uint32_t pos = block_excl_scan<NT>(sel, part, &tot_sel);
-            if (i < c) {
-                if (sel) {
-                    a.jobs[base + pos] = g;
-                } else {
-                    // Carry: move the activation record to the next parity and queue the tile
-                    // in list cur^1 for the wave after.  That list was emptied above (or is
-                    // still empty on the first boundary) and nothing else runs during the
-                    // boundary, so the tile is not in it (qwave < wave + 2), its shard slot
-                    // comes from an LDS counter and qwave is a plain store: no returned
-                    // global atomic sits on this path (activate() needs two in a row).
-                    uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
-                    uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
-                    const uint32_t mb = *mp;
-                    *kp = kInfBits;
-                    *mp = 0u;
-                    const size_t np = p ^ 1u;
-                    atomicMin(&a.key[np * a.tiles + g], k);
-                    atomicOr(&a.mask[np * a.tiles + g], mb);
-                    a.qwave[g] = wave + 2u;
-                    const uint32_t sh = g % kShards;
-                    const uint32_t slot = atomicAdd(&hist[sh], 1u);
-                    a.list[((size_t)(cur ^ 1u) * kShards + sh) * a.shard_cap + slot] = g;
+        const size_t np = p ^ 1u;
+        // Carry: move the activation record to the next parity and queue the tile in list
+        // cur^1 for the wave after.  That list was emptied above (or is still empty on the
+        // first boundary) and nothing else runs during the boundary, so the tile is not in it
+        // (qwave < wave + 2), its shard slot comes from an LDS counter and qwave is a plain
+        // store: no returned global atomic sits on this path (activate() needs two in a row).
+        auto carry = [&](uint32_t g, uint32_t k, uint32_t mb) {
+            a.key[(size_t)p * a.tiles + g] = kInfBits;
+            a.mask[(size_t)p * a.tiles + g] = 0u;
+            atomicMin(&a.key[np * a.tiles + g], k);
+            atomicOr(&a.mask[np * a.tiles + g], mb);
+            a.qwave[g] = wave + 2u;
+            const uint32_t sh = g % kShards;
+            const uint32_t slot = atomicAdd(&hist[sh], 1u);
+            a.list[((size_t)(cur ^ 1u) * kShards + sh) * a.shard_cap + slot] = g;
+        };
+        if (S && c > 2u * NT) {
+            // Long staged lists: each thread takes a contiguous run of `per` entries, so two
+            // block scans (ties, then selections) cover the whole list, and the carried tiles'
+            // border bits are fetched as one batch of loads before any of them is written.
+            // (Lists of up to two block passes keep the loop below, which times better there.)
+            constexpr int UM = STAGE > 0 ? (STAGE + NT - 1) / NT : 1;
+            const uint32_t per = (c + NT - 1) / NT;       // <= UM since c <= STAGE
+            const uint32_t i0 = (uint32_t)tid * per;
+            uint32_t ntie = 0;
+            for (uint32_t u = 0; u < per; ++u) {
+                const uint32_t i = i0 + u;
+                if (i < c) {
+                    const uint32_t k = sk[i];
+                    ntie += (k >= lo_base && k < thr_hi) ? 1u : 0u;
                 }
             }
-            base += tot_sel;
-            quota_used += tot_tie;
-            carried += (i0 + NT <= c ? NT : c - i0) - tot_sel;
+            uint32_t tot_tie;
+            uint32_t rank = block_excl_scan<NT>(ntie, part, &tot_tie);
+            uint32_t selbits = 0, nsel = 0;
+            for (uint32_t u = 0; u < per; ++u) {
+                const uint32_t i = i0 + u;
+                if (i < c) {
+                    const uint32_t k = sk[i];
+                    const bool low = k < lo_base;
+                    const bool tie = !low && k < thr_hi;
+                    if (low || (tie && rank < quota)) {
+                        selbits |= 1u << u;
+                        ++nsel;
+                    }
+                    rank += tie ? 1u : 0u;
+                }
+            }
+            uint32_t tot_sel;
+            uint32_t pos = block_excl_scan<NT>(nsel, part, &tot_sel);
+            uint32_t mbv[UM];
+#pragma unroll
+            for (int u = 0; u < UM; ++u) {
+                const uint32_t i = i0 + (uint32_t)u;
+                mbv[u] = 0u;
+                if ((uint32_t)u < per && i < c && !((selbits >> u) & 1u))
+                    mbv[u] = a.mask[(size_t)p * a.tiles + sg[i]];
+            }
+#pragma unroll
+            for (int u = 0; u < UM; ++u) {
+                const uint32_t i = i0 + (uint32_t)u;
+                if ((uint32_t)u < per && i < c) {
+                    const uint32_t g = sg[i];
+                    if ((selbits >> u) & 1u) a.jobs[pos++] = g;
+                    else carry(g, sk[i], mbv[u]);
+                }
+            }
+            base = tot_sel;
+            carried = c - tot_sel;
+        } else {
+            uint32_t quota_used = 0;
+            for (uint32_t i0 = 0; i0 < c; i0 += NT) {
+                const uint32_t i = i0 + tid;
+                uint32_t g = 0, k = 0xFFFFFFFFu, low = 0, tie = 0;
+                if (i < c) {
+                    g = entry_g(i);
+                    k = entry_k(i, g);
+                    low = k < lo_base ? 1u : 0u;
+                    tie = (!low && k < thr_hi) ? 1u : 0u;
+                }
+                uint32_t tot_tie;
+                const uint32_t tie_rank = block_excl_scan<NT>(tie, part, &tot_tie) + quota_used;
+                const uint32_t sel = low | ((tie && tie_rank < quota) ? 1u : 0u);
+                uint32_t tot_sel;
+                const uint32_t pos = block_excl_scan<NT>(sel, part, &tot_sel);
+                if (i < c) {
+                    if (sel) a.jobs[base + pos] = g;
+                    else carry(g, k, a.mask[(size_t)p * a.tiles + g]);
+                }
+                base += tot_sel;
+                quota_used += tot_tie;
+                carried += (i0 + NT <= c ? NT : c - i0) - tot_sel;
+            }
         }
     };
     if (staged) passes(std::true_type{});
