@@ -463,7 +463,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 hs[k++] = i;
             }
     }
-    if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, std::min<uint64_t>(n_sel, T))))) return rc;
+    if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, T)))) return rc;
     // Grid of the device-mode solve kernel: at most one resident round of work-groups;
     // each work-group loops over list entries, so the grid never has to match the list.
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);

@@ -69,7 +69,9 @@ struct GhcmCtrl {
     uint32_t abort;         // persistent loop: a barrier wait timed out
     uint32_t carried;       // tiles carried over by the top-n selection (all waves)
     uint32_t ticks;         // k_wave_end launches executed, the idle ones included
-    uint32_t pad0[4];
+    uint32_t n_carry;       // tiles a long selection left to the wave's tile launch to carry
+                            // (jobs[n_jobs, n_jobs + n_carry))
+    uint32_t pad0[3];
     uint32_t prefix[kShards + 1];   // exclusive prefix of the shard counts of list[cur]
     uint32_t pad1[31];
     uint32_t count[2][kShards * kCounterStride];

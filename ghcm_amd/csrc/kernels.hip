@@ -155,6 +155,25 @@ __device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32
     if (old < wave + 2u) list_append(a, cur ^ 1u, nb);
 }
 
+// Tiles the wave boundary did not select and left to this launch: each moves its activation
+// record to the next parity and is queued for the next wave through activate(), whose qwave
+// test keeps it from entering the list twice when this wave's own activations reach it
+// first.  Every thread of the grid takes a share; only unselected tiles are touched, and this
+// wave's claims and activations use other records, so nothing races.
+__device__ __forceinline__ void carry_handed_off(const SolveArgs& a, uint32_t nj, uint32_t ncarry,
+                                                 uint32_t wave, uint32_t cur, uint32_t p) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncarry; i += stride) {
+        const uint32_t g = a.jobs[nj + i];
+        uint32_t* kp = &a.key[(size_t)p * a.tiles + g];
+        uint32_t* mp = &a.mask[(size_t)p * a.tiles + g];
+        const uint32_t k = *kp, mb = *mp;
+        *kp = kInfBits;
+        *mp = 0u;
+        activate(a, g, wave, cur, p, mb, __uint_as_float(k));
+    }
+}
+
 // Entry i of the current wave list: shard found by a search over the prefix counts.
 __device__ __forceinline__ uint32_t list_entry(const SolveArgs& a, const uint32_t* prefix,
                                                uint32_t cur, uint32_t i) {
@@ -535,9 +554,11 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t ncarry = ctrl->n_carry;
         static_assert(kShards < 64, "one lane per prefix entry");
         const uint32_t pv = lane <= (int)kShards ? ctrl->prefix[lane] : 0u;
-        if (done || blockIdx.x >= nj) return;
+        // Work-groups past the jobs still take their share of the handed-off carry.
+        if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
         const uint32_t p = wave & 1u;
         if (lane <= (int)kShards) s_prefix[lane] = pv;
         __syncthreads();
@@ -550,6 +571,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
                                                     t_s, c_s, open_s, add_s);
             __syncthreads();
         }
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
     }
 }
 
@@ -868,10 +890,12 @@ void k_solve_sweep(SolveArgs a) {
         const uint32_t wave = ctrl->wave;
         const uint32_t cur = ctrl->cur;
         const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t ncarry = ctrl->n_carry;
         // Every lane reads a valid word (no branch around the load).
         const uint32_t pr = ctrl->prefix[lane & (int)(kShards - 1)];
         const uint32_t pv = lane < (int)kShards ? pr : 0xFFFFFFFFu;
-        if (done || blockIdx.x >= nj) return;
+        // Work-groups past the jobs still take their share of the handed-off carry.
+        if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
         const uint32_t p = wave & 1u;
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             uint32_t g;
@@ -888,6 +912,7 @@ void k_solve_sweep(SolveArgs a) {
             sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
     }
 }
 
@@ -932,7 +957,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, 
 // into jobs[]; the rest are carried to the next list with their key and border bits.
 // Returns the new current list, or -1 without writing anything when `stop_if_done` is set
 // and the loop is already done.
-template <int NT, int STAGE>
+template <int NT, int STAGE, bool HANDOFF = false>
 __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint32_t* part,
                            uint32_t* s_prefix, uint32_t* s_misc, uint32_t* sg, uint32_t* sk,
                            bool stop_if_done = false) {
@@ -984,6 +1009,7 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
         else ctrl->busy_waves = busy0 + 1u;
         ctrl->jobs_from_list = 1u;
         ctrl->n_jobs = c;
+        ctrl->n_carry = 0u;
     }
     if (c <= n) return (int)cur;
 
@@ -1001,6 +1027,7 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
     // Two copies of the histogram and compaction passes, one reading the LDS stage and one
     // the list in HBM: a runtime choice between the two would make every read a flat one.
     uint32_t base = 0, carried = 0;
+    bool handed = false;                 // carried tiles left to the tile launch (HANDOFF)
     auto passes = [&](auto staged_c) {
         constexpr bool S = decltype(staged_c)::value;
         auto entry_g = [&](uint32_t i) { return S ? sg[i] : list_entry(a, s_prefix, cur, i); };
@@ -1062,8 +1089,10 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
         };
         if (S && c > 2u * NT) {
             // Long staged lists: each thread takes a contiguous run of `per` entries, so two
-            // block scans (ties, then selections) cover the whole list, and the carried tiles'
-            // border bits are fetched as one batch of loads before any of them is written.
+            // block scans (ties, then selections) cover the whole list.  With HANDOFF the
+            // carried tiles are written after the selected ones for the tile launch to move
+            // (its whole grid does that in parallel); otherwise their border bits are fetched
+            // as one batch of loads and this work-group moves them.
             // (Lists of up to two block passes keep the loop below, which times better there.)
             constexpr int UM = STAGE > 0 ? (STAGE + NT - 1) / NT : 1;
             const uint32_t per = (c + NT - 1) / NT;       // <= UM since c <= STAGE
@@ -1094,21 +1123,34 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
             }
             uint32_t tot_sel;
             uint32_t pos = block_excl_scan<NT>(nsel, part, &tot_sel);
-            uint32_t mbv[UM];
+            if constexpr (HANDOFF) {
+                // Entries before this thread's run that were not selected come first.
+                uint32_t cpos = tot_sel + min(i0, c) - pos;
+                for (uint32_t u = 0; u < per; ++u) {
+                    const uint32_t i = i0 + u;
+                    if (i < c) {
+                        if ((selbits >> u) & 1u) a.jobs[pos++] = sg[i];
+                        else a.jobs[cpos++] = sg[i];
+                    }
+                }
+                handed = true;
+            } else {
+                uint32_t mbv[UM];
 #pragma unroll
-            for (int u = 0; u < UM; ++u) {
-                const uint32_t i = i0 + (uint32_t)u;
-                mbv[u] = 0u;
-                if ((uint32_t)u < per && i < c && !((selbits >> u) & 1u))
-                    mbv[u] = a.mask[(size_t)p * a.tiles + sg[i]];
-            }
+                for (int u = 0; u < UM; ++u) {
+                    const uint32_t i = i0 + (uint32_t)u;
+                    mbv[u] = 0u;
+                    if ((uint32_t)u < per && i < c && !((selbits >> u) & 1u))
+                        mbv[u] = a.mask[(size_t)p * a.tiles + sg[i]];
+                }
 #pragma unroll
-            for (int u = 0; u < UM; ++u) {
-                const uint32_t i = i0 + (uint32_t)u;
-                if ((uint32_t)u < per && i < c) {
-                    const uint32_t g = sg[i];
-                    if ((selbits >> u) & 1u) a.jobs[pos++] = g;
-                    else carry(g, sk[i], mbv[u]);
+                for (int u = 0; u < UM; ++u) {
+                    const uint32_t i = i0 + (uint32_t)u;
+                    if ((uint32_t)u < per && i < c) {
+                        const uint32_t g = sg[i];
+                        if ((selbits >> u) & 1u) a.jobs[pos++] = g;
+                        else carry(g, sk[i], mbv[u]);
+                    }
                 }
             }
             base = tot_sel;
@@ -1147,6 +1189,7 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
         ctrl->jobs_from_list = 0u;
         ctrl->n_jobs = base;
         ctrl->carried += carried;
+        if (handed) ctrl->n_carry = carried;
     }
     return (int)cur;
 }
@@ -1181,8 +1224,8 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
     const int tid = threadIdx.x;
     // Read together with the control words (one round trip); only this kernel writes it.
     const uint32_t ticks0 = a.ctrl->ticks;
-    const int cur = wave_select<1024, kSelectStage>(a, first != 0u, hist, part, s_prefix, s_misc,
-                                                    sg, sk, true);
+    const int cur = wave_select<1024, kSelectStage, true>(a, first != 0u, hist, part, s_prefix,
+                                                          s_misc, sg, sk, true);
     // An idle launch (the loop is done) only advances the tick, so the host still sees progress.
     if (tid == 0) a.ctrl->ticks = ticks0 + 1u;
     if (cur < 0 || !a.rec || !a.n_slots) return;
@@ -1792,8 +1835,10 @@ __global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8)))
     const uint32_t wave = ctrl->wave;
     const uint32_t cur = ctrl->cur;
     const uint32_t from_list = ctrl->jobs_from_list;
+    const uint32_t ncarry = ctrl->n_carry;
     const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
-    if (done || blockIdx.x >= nj) return;
+    // Work-groups past the jobs still take their share of the handed-off carry.
+    if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
     const uint32_t p = wave & 1u;
     if (tid <= (int)kShards) s_prefix[tid] = pv;
     __syncthreads();
@@ -1803,6 +1848,7 @@ __global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8)))
         solve3d_tile<KIND>(a, g, wave, cur, p, t_s, c_s, s_u32, s_f32, in0);
         __syncthreads();
     }
+    if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
 }
 
 // Seeds of 3-D solves: (x, y, z, instance) quadruples; T = 0 at the seed, and the tiles of

@@ -283,7 +283,7 @@ int ghcm_server::init() {
     if ((rc = tmin.reserve(4 * T))) return rc;
     if ((rc = list.reserve(8ull * kShards * shard_cap))) return rc;
     if ((rc = tstat.reserve(12 * T))) return rc;
-    if ((rc = sel.reserve(4ull * std::max<uint64_t>(1, std::min<uint64_t>(n_sel, T))))) return rc;
+    if ((rc = sel.reserve(4ull * std::max<uint64_t>(1, T)))) return rc;
     if ((rc = ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
     if ((rc = sgen.reserve(4ull * slots))) return rc;
     if ((rc = totals.reserve(3 * sizeof(unsigned long long)))) return rc;

@@ -114,3 +114,18 @@ def test_3d_volume_128_vs_fp64():
         t, st = ctx.solve3d(cost, seeds)
     assert st["converged"] == 1
     _check(t, ref, cost, seeds, ctx.config.min_change)
+
+
+def test_3d_long_lists_carried_by_the_tile_launch():
+    """The 256^3 bench volume at n = 4096: its boundaries select from lists of 4-6 thousand
+    tiles and hand the unselected ones to the tile launch, which carries them to the next
+    wave; nothing may be lost or queued twice."""
+    from ghcm_amd import workloads
+    speed, seeds = workloads.volume_256()
+    cost = (np.float32(1.0) / speed).astype(np.float32)
+    ref = oracle.fmm3d_f64(cost.astype(np.float64), seeds)
+    print("case 3d long lists 256^3 n=4096", flush=True)
+    with g.Context(0, n_active=4096) as ctx:
+        t, st = ctx.solve3d(cost, seeds)
+    assert st["converged"] == 1
+    _check(t, ref, cost, seeds, ctx.config.min_change)

@@ -87,6 +87,38 @@ def test_top_n_sizes(rings256, n_active, sweep):
     check(t, ref32, seeds, cost, label=f"n={n_active} sweep={sweep}")
 
 
+def test_long_lists_carried_by_the_tile_launch():
+    """A 2048^2 ring field in 8x8 tiles at n = 64: lists grow to thousands of tiles, so the
+    boundaries hand the unselected tiles to the tile launch, which carries them to the next
+    wave; nothing may be lost or queued twice."""
+    from ghcm_amd import workloads
+    r = workloads._make(2048, 2048, "ring", 77, 4, 78)
+    ref = oracle.fmm_f32(r.cost, r.seeds)
+    print("case long lists systolic 8x8 2048^2 n=64", flush=True)
+    with g.Context(0, tile_w=8, tile_h=8, kernel=g.KERNEL_SYSTOLIC, n_active=64) as ctx:
+        t, st = ctx.solve(r.cost, r.seeds)
+    assert st["converged"] == 1
+    check(t, ref, r.seeds, r.cost, label="long lists systolic")
+
+
+def test_long_lists_carried_in_a_sweep_batch():
+    """The same with the sweep kernel: 16 ring instances of 1024^2 in 32x32 tiles, n = 64."""
+    from ghcm_amd import workloads
+    reqs = [workloads._make(1024, 1024, "ring", 300 + i, 1 + i % 4, 400 + i) for i in range(16)]
+    print("case long lists sweep batch 16 x 1024^2 n=64", flush=True)
+    with g.Context(0, tile_w=32, tile_h=32, kernel=g.KERNEL_SWEEP, n_active=64,
+                   inner_iter_cap=1) as ctx:
+        dev = [(ctx.to_device(r.cost), ctx.alloc((r.h, r.w), np.float32)) for r in reqs]
+        st = ctx.solve_batch(g.FIELD_COST_F32, [(dc.ptr, dt.ptr, r.w, r.h, r.seeds)
+                                                for (dc, dt), r in zip(dev, reqs)])
+        assert st["converged"] == 1
+        for (dc, dt), r in zip(dev, reqs):
+            check(dt.download(), oracle.fmm_f32(r.cost, r.seeds), r.seeds, r.cost,
+                  label="long lists sweep batch")
+            dc.free()
+            dt.free()
+
+
 SHAPES = [(1, 1), (1, 37), (37, 1), (13, 29), (100, 70), (3, 250), (257, 129)]
 
 
