@@ -1089,8 +1089,9 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
         };
         if (S && c > 2u * NT) {
             // Long staged lists: each thread takes a contiguous run of `per` entries, so two
-            // block scans (ties, then selections) cover the whole list.  With HANDOFF the
-            // carried tiles are written after the selected ones for the tile launch to move
+            // block scans (ties, then selections) cover the whole list.  With HANDOFF (after
+            // a solve's first boundary) the carried tiles are written after the selected ones
+            // for the tile launch to move
             // (its whole grid does that in parallel); otherwise their border bits are fetched
             // as one batch of loads and this work-group moves them.
             // (Lists of up to two block passes keep the loop below, which times better there.)
@@ -1123,7 +1124,9 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
             }
             uint32_t tot_sel;
             uint32_t pos = block_excl_scan<NT>(nsel, part, &tot_sel);
-            if constexpr (HANDOFF) {
+            // Not on a solve's first boundary: the persistent and fused forms start with one,
+            // and their tile loops do not take handed-off tiles.
+            if (HANDOFF && !first) {
                 // Entries before this thread's run that were not selected come first.
                 uint32_t cpos = tot_sel + min(i0, c) - pos;
                 for (uint32_t u = 0; u < per; ++u) {

@@ -101,6 +101,27 @@ def test_long_lists_carried_by_the_tile_launch():
     check(t, ref, r.seeds, r.cost, label="long lists systolic")
 
 
+@pytest.mark.parametrize("launch_mode", [0, 1, 3])
+def test_long_first_list_in_every_launch_mode(launch_mode):
+    """Every 32x32 tile of a 2048^2 field is a walled chamber (cost +INF on its last row and
+    column) with a source at its centre: the first boundary selects 64 of 4,096 queued tiles,
+    and a tile it does not carry is never queued again, since no neighbour can reach it.  The
+    persistent (1) and fused (3) forms start with that boundary and their tile loops take no
+    handed-off tiles, so the boundary must carry the unselected ones itself."""
+    w = h = 2048
+    cost = g.cost_from_speed(g.gen_smooth_speed(w, h, 91, 32.0, 0.1, 10.0))
+    cost[31::32, :] = np.inf
+    cost[:, 31::32] = np.inf
+    seeds = [(32 * i + 15, 32 * j + 15) for j in range(h // 32) for i in range(w // 32)]
+    ref = oracle.fmm_f32(cost, seeds)
+    print(f"case walled chambers launch_mode={launch_mode}", flush=True)
+    with g.Context(0, tile_w=32, tile_h=32, kernel=g.KERNEL_SWEEP, n_active=64, inner_iter_cap=1,
+                   launch_mode=launch_mode) as ctx:
+        t, st = ctx.solve(cost, seeds)
+    assert st["converged"] == 1
+    check(t, ref, seeds, cost, label=f"walled chambers launch_mode={launch_mode}")
+
+
 def test_long_lists_carried_in_a_sweep_batch():
     """The same with the sweep kernel: 16 ring instances of 1024^2 in 32x32 tiles, n = 64."""
     from ghcm_amd import workloads
