@@ -21,8 +21,10 @@
 //
 // Scheduling (device mode): a wave's tiles sit in list[cur]; k_wave_end (one
 // work-group) picks the n lowest keys with a two-level radix histogram and k_solve
-// solves them, carrying the rest to the next list.  Activations append neighbours to
-// list[cur^1] once per wave (atomicMax on qwave), sharded over kShards counters.
+// solves them.  The rest go to the next list: the boundary moves them itself, except for
+// long lists after a solve's first boundary, whose unselected tiles it leaves to the tile
+// launch (carry_handed_off).  Activations append neighbours to list[cur^1] once per wave
+// (atomicMax on qwave), sharded over kShards counters.
 // Host mode takes an explicit job list and writes one status record per job
 // (src/eikonal_hcm.cpp:396-420).
 #include <hip/hip_ext.h>
