@@ -445,7 +445,8 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     if ((rc = ctx->mask.reserve(8ull * T))) return rc;
     if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
     const uint32_t shard_cap = (T + kShards - 1) / kShards;
-    if ((rc = ctx->list.reserve(8ull * kShards * shard_cap))) return rc;
+    // three lists: boundary-free waves rotate them (the other modes use two)
+    if ((rc = ctx->list.reserve(12ull * kShards * shard_cap))) return rc;
     if ((rc = ctx->tstat.reserve(12ull * T))) return rc;
     if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
     if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
@@ -534,7 +535,62 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     const bool fused = !three_d && !host_sched && cfg.launch_mode == 3 && fk.launch != nullptr;
     const PersistentKernel pk = get_persistent_kernel((int)tw, (int)th, kind, cfg.kernel);
     const bool persistent = !three_d && !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
-    if (persistent) {
+    // launch_mode 4: boundary-free waves (sweep kernel): one tile launch per wave.
+    const BfKernel bk = get_bf_kernel((int)tw, (int)th, kind, cfg.kernel, wide);
+    const bool bfree = !three_d && !host_sched && cfg.launch_mode == 4 && bk.launch != nullptr;
+    if (bfree) {
+        // ---- boundary-free device wave loop ----
+        // Each tile launch reads its own list's counts; its first work-group publishes
+        // (done << 32) | (wave + 1) into the mapped progress record.  The host keeps `lead`
+        // launches enqueued ahead of the last one started and stops at the first wave that
+        // found its list empty; the launches already enqueued behind it exit at once.
+        a.rec = ctx->rec.dev_as<HostRec>();
+        const bool prof = cfg.profile_kernels != 0;
+        const uint32_t lead = std::max<uint32_t>(2, cfg.waves_per_poll ? cfg.waves_per_poll : 4);
+        const uint32_t bresident = (uint32_t)std::max(1, bk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
+        const uint32_t bgrid = std::max(1u, std::min(n_sel, bresident));
+        uint64_t enqueued = 0;
+        for (uint64_t spins = 0;; ++spins) {
+            const unsigned long long progress = hr->progress;
+            if (progress >> 32) { converged = true; break; }
+            if (enqueued >= max_waves) break;
+            const uint64_t started = (uint32_t)progress;
+            if (enqueued < started + lead) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (prof) {
+                    const size_t need = 2 * (enqueued + 1);
+                    while (ctx->ev_prof.size() < need) {
+                        hipEvent_t ev;
+                        HIP_TRY(hipEventCreate(&ev));
+                        ctx->ev_prof.push_back(ev);
+                    }
+                    e0 = ctx->ev_prof[2 * enqueued];
+                    e1 = ctx->ev_prof[2 * enqueued + 1];
+                }
+                HIP_TRY(bk.launch(a, (uint32_t)enqueued, bgrid, st, e0, e1));
+                ++enqueued;
+                continue;
+            }
+            if ((spins & 4095u) == 4095u) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady)
+                    return fail(GHCM_ERR_HIP, "wave loop: %s", hipGetErrorString(q));
+            }
+            __builtin_ia32_pause();
+        }
+        HIP_TRY(hipEventRecord(ctx->ev_end, st));
+        if ((rc = publish_totals())) return rc;
+        if (prof) {
+            for (uint64_t k = 0; k < enqueued; ++k) {
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_prof[2 * k], ctx->ev_prof[2 * k + 1]));
+                kern_us += 1000.0 * ms;
+                ++kern_launches;
+            }
+        }
+        converged = (hr->progress >> 32) != 0;
+        waves_done = hr->busy_waves;
+    } else if (persistent) {
         // ---- one cooperative launch runs every wave (grid = one resident round) ----
         const int per_cu = pk.blocks_per_cu();
         uint32_t pgrid = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, ctx->cus);

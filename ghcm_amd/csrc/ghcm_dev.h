@@ -13,11 +13,13 @@
 //     tinst[g]       u32  instance index (batches only)
 //     tstat[3][g]    u32  visits / in-tile passes / improved cells of tile g
 //                         (owner-only increments; summed once per solve)
-//   list[2][kShards][shard_cap]  u32  wave lists, sharded by g % kShards so list
+//   list[3][kShards][shard_cap]  u32  wave lists, sharded by g % kShards so list
 //                         appends spread over kShards counters (each on its own
 //                         128-byte line) instead of contending on one address.
-//                         list[cur] is read by wave w; list[cur^1] collects the
-//                         activations for wave w+1.
+//                         With a wave-boundary kernel, list[cur] is read by wave w and
+//                         list[cur^1] collects the activations for wave w+1 (two lists).
+//                         Boundary-free waves rotate three: wave w reads list[w % 3],
+//                         appends to list[(w+1) % 3] and empties list[(w+2) % 3].
 //   ctrl             GhcmCtrl, one per solve
 #pragma once
 
@@ -74,7 +76,7 @@ struct GhcmCtrl {
     uint32_t pad0[3];
     uint32_t prefix[kShards + 1];   // exclusive prefix of the shard counts of list[cur]
     uint32_t pad1[31];
-    uint32_t count[2][kShards * kCounterStride];
+    uint32_t count[3][kShards * kCounterStride];
     uint32_t arrive_top;
     uint32_t pad2[31];
     uint32_t arrive[kMaxBarGroups * kCounterStride];
@@ -122,11 +124,12 @@ struct SolveArgs {
     uint32_t*       key;     // [2][tiles]
     uint32_t*       mask;    // [2][tiles]
     float*          tmin;
-    uint32_t*       list;    // [2][kShards][shard_cap]
+    uint32_t*       list;    // [3][kShards][shard_cap] (the first two with a boundary kernel)
     uint32_t        tiles;
     uint32_t        shard_cap;
     uint32_t*       tstat;   // [3][tiles]
-    uint32_t*       jobs;    // compacted top-n selection, capacity min(n, tiles)
+    uint32_t*       jobs;    // compacted top-n selection followed by the tiles a long selection
+                             // hands to the tile launch: capacity `tiles` words
     HostRec*        rec;     // device view of the progress record (null: not written)
     const uint32_t* sgen;    // serving: admission generation per slot
     uint32_t        n_slots; // serving: instance slots (<= kMaxServeSlots), 0 otherwise

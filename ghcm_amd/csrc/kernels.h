@@ -27,6 +27,15 @@ constexpr uint32_t kTile3D = 8;
 // Tile solve with the wave boundary folded in (one launch per wave); sweep kernel only.
 SolveKernel get_fused_kernel(int tile_w, int tile_h, int kind, int mode);
 
+// Boundary-free waves (sweep kernel): one tile launch per wave, the wave index as argument; no
+// wave-boundary kernel and no top-n cut (every queued tile is solved in its wave).
+struct BfKernel {
+    hipError_t (*launch)(const SolveArgs&, uint32_t wave, uint32_t grid, hipStream_t,
+                         hipEvent_t ev0, hipEvent_t ev1);
+    int (*blocks_per_cu)();
+};
+BfKernel get_bf_kernel(int tile_w, int tile_h, int kind, int mode, bool wide);
+
 // Persistent device wave loop (cooperative launch); null when not built for the combination.
 struct PersistentKernel {
     hipError_t (*launch)(const SolveArgs&, uint32_t grid, hipStream_t);

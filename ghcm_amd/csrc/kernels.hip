@@ -146,15 +146,16 @@ __device__ __forceinline__ void list_append(const SolveArgs& a, uint32_t which, 
 }
 
 // Queue tile nb for the next wave unless it is already queued, recording why (border
-// side bit) and the smallest candidate time that activated it.
+// side bit) and the smallest candidate time that activated it.  `nxt` is the list that
+// collects the next wave's tiles.
 __device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32_t wave,
-                                         uint32_t cur, uint32_t p, uint32_t mask_bits,
+                                         uint32_t nxt, uint32_t p, uint32_t mask_bits,
                                          float key) {
     const uint32_t np = p ^ 1u;
     atomicMin(&a.key[(size_t)np * a.tiles + nb], __float_as_uint(key));
     atomicOr(&a.mask[(size_t)np * a.tiles + nb], mask_bits);
     uint32_t old = atomicMax(&a.qwave[nb], wave + 2u);
-    if (old < wave + 2u) list_append(a, cur ^ 1u, nb);
+    if (old < wave + 2u) list_append(a, nxt, nb);
 }
 
 // Tiles the wave boundary did not select and left to this launch: each moves its activation
@@ -163,7 +164,7 @@ __device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32
 // first.  Every thread of the grid takes a share; only unselected tiles are touched, and this
 // wave's claims and activations use other records, so nothing races.
 __device__ __forceinline__ void carry_handed_off(const SolveArgs& a, uint32_t nj, uint32_t ncarry,
-                                                 uint32_t wave, uint32_t cur, uint32_t p) {
+                                                 uint32_t wave, uint32_t nxt, uint32_t p) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncarry; i += stride) {
         const uint32_t g = a.jobs[nj + i];
@@ -172,7 +173,7 @@ __device__ __forceinline__ void carry_handed_off(const SolveArgs& a, uint32_t nj
         const uint32_t k = *kp, mb = *mp;
         *kp = kInfBits;
         *mp = 0u;
-        activate(a, g, wave, cur, p, mb, __uint_as_float(k));
+        activate(a, g, wave, nxt, p, mb, __uint_as_float(k));
     }
 }
 
@@ -279,7 +280,7 @@ __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, i
 
 template <int TW, int TH, int KIND, int MODE, bool HOST>
 __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uint32_t wave,
-                             uint32_t cur, uint32_t p, uint32_t job_slot,
+                             uint32_t nxt, uint32_t p, uint32_t job_slot,
                              float* t_s, float* c_s, uint8_t* open_s, uint8_t* add_s) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
@@ -517,14 +518,14 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             stat_add(&a.tstat[g], 1u);
             stat_add(&a.tstat[(size_t)a.tiles + g], passes);
             stat_add(&a.tstat[2 * (size_t)a.tiles + g], improved);
-            if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tmin);
+            if (unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tmin);
         }
         if (lane < 4 && ((side_flags >> lane) & 1u)) {
             const int s = lane;
             int nx = tx + (s == 0 ? -1 : s == 2 ? 1 : 0);
             int ny = ty + (s == 1 ? -1 : s == 3 ? 1 : 0);
             uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
-            activate(a, nb, wave, cur, p, 1u << (s ^ 2), side_key[s]);
+            activate(a, nb, wave, nxt, p, 1u << (s ^ 2), side_key[s]);
         }
     }
 }
@@ -569,11 +570,11 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
             uint32_t mb = 0;
             if (lane == 0) mb = claim_tile(a, g, p);
             mb = __shfl(mb, 0);
-            process_tile<TW, TH, KIND, MODE, false>(a, g, mb, wave, cur, p, j,
+            process_tile<TW, TH, KIND, MODE, false>(a, g, mb, wave, cur ^ 1u, p, j,
                                                     t_s, c_s, open_s, add_s);
             __syncthreads();
         }
-        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
     }
 }
 
@@ -603,8 +604,108 @@ __device__ __forceinline__ float dpp_from_lower_lane(float old, float v) {
 template <int TW, int TH>
 constexpr int sweep_lds_words() { return 2 * (TW + 2) * (TH + 2) + 1; }
 
+// One round of one sweep order over the LDS tile (time box at shb, cost box CB words after it),
+// run by one wave64.  DX > 0: lane L owns column x = L, sweeping towards +x; DX < 0: column
+// TW-1-L, towards -x.  DOWN: rows ascending.  Wrapped skew: at step s lane L relaxes pass
+// position (s - L) mod TH of its column (row p of a downward pass, TH-1-p of an upward one),
+// so lane L reaches the first row of its ordered pass at step L, one step after lane L-1, and
+// the lanes' ordered passes form one anti-diagonal front; TW + TH - 1 steps give every column
+// one complete ordered pass, and before and after it a lane relaxes cells of the neighbouring
+// passes instead of idling (every value involved is an upper bound).
+//
+// A step reads only the cell's two upwind neighbours and its cost: the sweep towards +x and +y
+// updates a cell from its -x and -y neighbours.  The other two sweeps' updates cover the +x and
+// +y neighbours, and the minimum of the four one-sided updates is the two-sided Godunov update
+// (it is monotone in both axis values), so the four sweeps together have the same fixed point
+// as sweeps that read all four neighbours.  The upwind values just computed arrive through
+// `prev` (this lane's previous cell, the upwind row) and a wave_shr:1 DPP move of lane L-1's
+// `prev` (the upwind column); each is merged with this lane's own LDS read of that cell, so
+// improvements made by the other three sweeps are used as soon as they land.  Lane 0 has no
+// source lane: `lreg` keeps +INF there and its upwind column is the halo read from LDS.
+// Results are merged into the tile with an LDS atomic minimum (no return value waited for).
+template <int TW, int TH, int DX, bool DOWN>
+__device__ __forceinline__ void sweep_round(char* const shb, const int L, float& lreg) {
+    constexpr int PW = TW + 2;
+    constexpr int PH = TH + 2;
+    constexpr int CB = PW * PH + 1;
+    static_assert((TH & (TH - 1)) == 0, "TH must be a power of two");
+    const bool lane_on = L < TW;
+    // Lanes past the tile width (TW < 64) shadow the last column and never store.
+    const int xc = lane_on ? (DX > 0 ? L : TW - 1 - L) : (DX > 0 ? TW - 1 : 0);
+    // Word offsets from the corner (r, xc) of the 3 x 3 box around the cell, which sits at
+    // padded position (r + 1, xc + 1): every read and the store are immediate offsets from one
+    // per-step address.
+    constexpr int kUpRow = DOWN ? 1 : 2 * PW + 1;
+    constexpr int kUpCol = DX > 0 ? PW : PW + 2;
+    constexpr int kCell = PW + 1;
+    constexpr int kCost = CB + PW + 1;
+    constexpr int b_step = DOWN ? 4 * PW : -4 * PW;
+    constexpr int r_first = DOWN ? 0 : TH - 1;
+    const int p0 = (-L) & (TH - 1);
+    const int r_start = DOWN ? p0 : TH - 1 - p0;
+    const int b_first = r_first * (4 * PW) + 4 * xc;
+    const int b_start = r_start * (4 * PW) + 4 * xc;
+    // Lane L is at the first row of its pass at the steps s with s = L (mod TH): a scalar
+    // one-hot pattern shifted by s & (TH-1) marks those lanes, and the row's byte offset
+    // advances by one row per step, back to the first row exactly there (the wrap of the row
+    // index coincides with the pass start).
+    constexpr uint64_t kFirstPat = TH >= 64 ? 1ull : ~0ull / ((1ull << (TH & 63)) - 1ull);
+    constexpr int S = TW + TH - 1;
+    constexpr int SE = (S + 1) & ~1;
+    int bb = b_start - b_step;
+    uint32_t jl = 0;
+    struct Nb {
+        float ur, uc, cc;
+        int b;
+        bool first;
+    };
+    auto at = [&](int b, int word) -> float { return *reinterpret_cast<const float*>(shb + b + 4 * word); };
+    // LDS reads of step s+1 are issued before step s is computed (two register sets): their
+    // cells are upper bounds whatever the interleaving, and the fresh upwind values arrive
+    // through DPP and `prev`, so the dependent chain per step is DPP + update.
+    auto load = [&](Nb& o) {
+        o.first = __builtin_amdgcn_inverse_ballot_w64(kFirstPat << (jl & (TH - 1)));
+        ++jl;
+        bb = o.first ? b_first : bb + b_step;
+        o.b = bb;
+        o.ur = at(o.b, kUpRow);
+        o.uc = at(o.b, kUpCol);
+        o.cc = at(o.b, kCost);
+    };
+    // Times are non-negative or +INF, so their bit patterns order like the values: the axis
+    // minima and the merge use integer minimum (no NaN canonicalisation needed).
+    auto umin = [](float u, float v) {
+        return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
+    };
+    float prev = __builtin_inff();
+    auto step = [&](const Nb& n) {
+        // Lane L-1's fresh value of the upwind column cell.
+        lreg = dpp_from_lower_lane(lreg, prev);
+        const float ax = umin(n.uc, lreg);
+        const float ay = umin(n.ur, n.first ? __builtin_inff() : prev);
+        const float t = godunov_raw(ax, ay, umin(ax, ay), n.cc);
+        if (TW == 64 || lane_on)
+            __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * kCell),
+                                   __float_as_uint(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // The raw result is handed on, not min(t, old): both consumers take the minimum with
+        // their own LDS read of the cell.
+        prev = t;
+    };
+    Nb A, B;
+    load(A);
+#pragma unroll 2
+    for (int st = 0; st < SE; st += 2) {
+        load(B);
+        __builtin_amdgcn_sched_barrier(0);
+        step(A);
+        load(A);
+        __builtin_amdgcn_sched_barrier(0);
+        step(B);
+    }
+}
+
 template <int TW, int TH, int KIND, bool HOST>
-__device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
+__device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t nxt,
                            uint32_t p, uint32_t job_slot, float* sh,
                            uint32_t* s_u32, float* s_f32, GhcmInst in_single) {
     constexpr int PW = TW + 2;
@@ -642,126 +743,48 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
 
     const float eps = a.eps;
     const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kSweepHardRounds) ? kSweepHardRounds : a.iter_cap;
-    const int dx = (w & 1) ? -1 : 1;
-    const bool down = (w & 2) == 0;                // rows ascending
-    const int x = (dx > 0) ? L : TW - 1 - L;
-    const bool lane_on = L < TW;
-    // Lanes past the tile width (TW < 64) shadow the last column and never store.
-    const int xc = lane_on ? x : ((dx > 0) ? TW - 1 : 0);
-    // Wrapped skew: at step s lane L handles pass position (s - L) mod TH of its column, i.e.
-    // row (s - L) mod TH of a downward sweep or TH-1 minus it of an upward one.  The upwind
-    // column neighbour of that row was produced by lane L-1 one step earlier and arrives
-    // through DPP; the upwind row neighbour is the lane's own previous result, except at the
-    // first row of a pass, whose upwind neighbour is the halo in LDS.  No lane ever idles:
-    // before and after its ordered pass a lane relaxes cells of the neighbouring passes,
-    // which is valid because every value involved is an upper bound.  TW + TH - 1 steps give
-    // every column one complete ordered pass.
-    static_assert((TH & (TH - 1)) == 0, "TH must be a power of two");
-    const int r_first = down ? 0 : TH - 1;
-    const int step_dir = down ? 1 : -1;
-    // Every round starts from the same positions: lane L reaches pass position 0 at step L,
-    // one step after lane L-1, so the ordered passes of the lanes form one sweep front.
-    const int p0 = (-L) & (TH - 1);
-    const int r_start = down ? p0 : TH - 1 - p0;
+    char* const shb = reinterpret_cast<char*>(sh);
     uint32_t rounds = 0;
     bool unfinished = false;
-    // LDS reads of step s+1 are issued before step s is computed (two register sets): their
-    // cells are upper bounds whatever the interleaving, and the fresh upwind values arrive
-    // through DPP and `prev`, so the dependent chain per step is DPP + update.
-    constexpr int S = TW + TH - 1;
-    constexpr int SE = (S + 1) & ~1;
-    // Byte offsets into the LDS tile: corner of the 3x3 around (r+1, xc+1), so every read
-    // and the store are immediate offsets from one address.
-    char* const shb = reinterpret_cast<char*>(sh);
-    const int xc4 = 4 * xc;
-    // Lane L is at the first row of its pass at the steps s with s = L (mod TH): a scalar
-    // one-hot pattern shifted by s & (TH-1) marks those lanes, and the row's byte offset
-    // advances by one row per step, back to the first row exactly there (the wrap of the row
-    // index coincides with the pass start).  This keeps the row bookkeeping out of the
-    // vector ALU: two operations per step instead of four.
-    constexpr uint64_t kFirstPat = TH >= 64 ? 1ull : ~0ull / ((1ull << (TH & 63)) - 1ull);
-    const int b_step = step_dir * (4 * PW);
-    const int b_first = r_first * (4 * PW) + xc4;
-    const int b_start = r_start * (4 * PW) + xc4;
-    int bb = b_start - b_step;
-    uint32_t jl = 0;
-    struct Nb {
-        float tc, cc, tl, tr, tu, td;
-        int b;
-        bool first;
-    };
-    auto at = [&](int b, int word) -> float { return *reinterpret_cast<const float*>(shb + b + 4 * word); };
-    auto load = [&](Nb& o) {
-        o.first = __builtin_amdgcn_inverse_ballot_w64(kFirstPat << (jl & (TH - 1)));
-        ++jl;
-        bb = o.first ? b_first : bb + b_step;
-        o.b = bb;
-        o.tu = at(o.b, 1);
-        o.tl = at(o.b, PW);
-        o.tc = at(o.b, PW + 1);
-        o.tr = at(o.b, PW + 2);
-        o.td = at(o.b, 2 * PW + 1);
-        o.cc = at(o.b, CB + PW + 1);
-    };
-    // Destination of the DPP move, carried from step to step: lane 0 has no source lane, is
-    // never written and keeps +INF (its left neighbour is the halo, read from LDS as n.tl),
-    // so no copy of an `old` operand is needed per step.
+    // Convergence: a round counts as changing the tile when some cell ends it at least eps
+    // (strictly: an improvement of 0 never counts, so eps = 0 terminates) below where it began;
+    // each thread compares its cells with a snapshot taken at the round's start.
+    float snap[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) snap[k] = told[k];
+    // Destination of the DPP move, carried from step to step and round to round: lane 0 has no
+    // source lane, is never written and keeps +INF.
     float lreg = __builtin_inff();
     for (;;) {
-        float prev = __builtin_inff();
-        float maxdec = 0.0f;
-        bb = b_start - b_step;
-        jl = 0;
-        // Times are non-negative or +INF, so their bit patterns order like the values: the
-        // axis minima and the merge use integer minimum (no NaN canonicalisation needed).
-        auto umin = [](float u, float v) {
-            return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
-        };
-        auto step = [&](const Nb& n) {
-            // Lane L-1's fresh value of the left cell.
-            lreg = dpp_from_lower_lane(lreg, prev);
-            const float ax = umin(umin(n.tl, lreg), n.tr);
-            const float up = n.first ? __builtin_inff() : prev;
-            const float ay = umin(umin(n.tu, up), n.td);
-            const float t = godunov_raw(ax, ay, umin(ax, ay), n.cc);
-            // Merge into the cell with an LDS min (no return value is waited for): the four
-            // sweeps write the same tile concurrently and a min never raises a cell another
-            // sweep has just lowered.  Non-negative floats order like their bit patterns.
-            if (TW == 64 || lane_on)
-                __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * (PW + 1)),
-                                       __float_as_uint(t), __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-            // Largest decrease (negative when t is not below the old value; NaN, from two
-            // +INF minima, is ignored by fmaxf).
-            maxdec = fmaxf(maxdec, n.tc - t);
-            // The raw result is handed on, not min(t, old): both consumers (lane L+1 next step
-            // as its left neighbour, this lane next step as its upper one) take the minimum
-            // with their own LDS read of the cell, issued after this lane's read of it in the
-            // same wave and therefore no larger than the old value seen here.
-            prev = t;
-        };
-        Nb A, B;
-        load(A);
-#pragma unroll 2
-        for (int st = 0; st < SE; st += 2) {
-            load(B);
-            __builtin_amdgcn_sched_barrier(0);
-            step(A);
-            load(A);
-            __builtin_amdgcn_sched_barrier(0);
-            step(B);
+        // Wave w runs sweep order w (the branch is wave-uniform).
+        switch (w) {
+            case 0: sweep_round<TW, TH, 1, true>(shb, L, lreg); break;
+            case 1: sweep_round<TW, TH, -1, true>(shb, L, lreg); break;
+            case 2: sweep_round<TW, TH, 1, false>(shb, L, lreg); break;
+            default: sweep_round<TW, TH, -1, false>(shb, L, lreg); break;
         }
         ++rounds;
-        // A strict improvement of at least eps (eps may be 0).
-        const bool any = __ballot(lane_on && maxdec > 0.0f && maxdec >= eps) != 0ull;
+        __syncthreads();
+        bool chg = false;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int i = tid + kSweepThreads * k;
+            if (i < P) {
+                const float now = t_s[(i / TW + 1) * PW + (i % TW) + 1];
+                const float d = snap[k] - now;      // NaN (INF - INF) counts as no change
+                if (d > 0.0f && d >= eps) chg = true;
+                snap[k] = now;
+            }
+        }
+        const bool any = __ballot(chg) != 0ull;
         if (L == 0) s_u32[w] = any ? 1u : 0u;
         __syncthreads();
+        // No third barrier: a wave writes s_u32 again only after the next round's first
+        // barrier, which every wave reaches after reading it here.
         const bool more = (s_u32[0] | s_u32[1] | s_u32[2] | s_u32[3]) != 0u;
-        __syncthreads();
         if (!more) break;
         if (rounds >= cap) { unfinished = true; break; }
     }
-
     // Commit improved cells first (plain stores: one writer per tile per wave), so the stores
     // drain while the border check and the activations run; the tile min is reduced after.
     char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
@@ -828,7 +851,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             const int nx = tx + (sd == 0 ? -1 : sd == 2 ? 1 : 0);
             const int ny = ty + (sd == 1 ? -1 : sd == 3 ? 1 : 0);
             const uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
-            activate(a, nb, wave, cur, p, 1u << (sd ^ 2), s_f32[sd]);
+            activate(a, nb, wave, nxt, p, 1u << (sd ^ 2), s_f32[sd]);
         }
     }
 
@@ -859,7 +882,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             st.requeue = unfinished ? 1u : 0u;
             a.status[job_slot] = st;
         } else {
-            if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tm);
+            if (unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
         }
     }
 }
@@ -911,10 +934,72 @@ void k_solve_sweep(SolveArgs a) {
                 g = a.jobs[j];
             }
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
-        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Boundary-free waves (launch_mode 4): every tile queued for a wave is solved in that wave,
+// so no selection runs and no wave-boundary kernel sits between two tile launches.  Wave w
+// (the launch argument) reads list[w % 3], whose shard counts are final because wave w-1
+// ended with its launch; every wave64 scans the counts in its lanes and finds a job's shard
+// with one ballot.  Activations go to list[(w+1) % 3].  The first work-group empties
+// list[(w+2) % 3] (consumed by wave w-1, filled by wave w+1), publishes progress for the
+// host, which keeps a few waves enqueued ahead and stops at the first empty one, and keeps
+// the wave counters.  The grid (n work-groups) loops over the list, so a wave of any length
+// is solved whole.
+// ---------------------------------------------------------------------------------------
+template <int TW, int TH, int KIND, int WPE = 1>
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_solve_sweep_bf(SolveArgs a, uint32_t wave) {
+    __shared__ float sh[sweep_lds_words<TW, TH>()];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const GhcmInst in0 = single_inst(a);
+    GhcmCtrl* ctrl = a.ctrl;
+    const uint32_t cur = wave % 3u;
+    const uint32_t nxt = cur == 2u ? 0u : cur + 1u;
+    const uint32_t emp = nxt == 2u ? 0u : nxt + 1u;
+    static_assert(kShards == 32, "one lane per shard, half a wave");
+    // Shard counts of this wave's list: lane l < 32 holds shard l (every lane reads a valid
+    // word, so the load needs no branch).
+    const uint32_t cnt = ctrl->count[cur][(uint32_t)(lane & (int)(kShards - 1)) * kCounterStride];
+    uint32_t inc = lane < (int)kShards ? cnt : 0u;
+#pragma unroll
+    for (int off = 1; off < (int)kShards; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    const uint32_t c = __shfl(inc, (int)kShards - 1);
+    // Exclusive prefix of the shard counts in lanes 0-31; lanes 32-63 never match a job.
+    const uint32_t pv = lane < (int)kShards ? inc - cnt : 0xFFFFFFFFu;
+    if (blockIdx.x == 0) {
+        if (tid < (int)kShards) ctrl->count[emp][tid * kCounterStride] = 0u;
+        if (tid == 0) {
+            ctrl->wave = wave;
+            if (c) {
+                __hip_atomic_fetch_add(&ctrl->busy_waves, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                ctrl->done = 1u;
+            }
+            if (a.rec) sys_store64(&a.rec->progress, ((unsigned long long)(c == 0u) << 32) | (wave + 1u));
+        }
+    }
+    if (c == 0u) return;
+    const uint32_t p = wave & 1u;
+    for (uint32_t j = blockIdx.x; j < c; j += gridDim.x) {
+        // Entry j of list[cur]: its shard is the last one whose prefix is <= j.
+        const uint32_t shd = (uint32_t)__popcll(__ballot(pv <= j)) - 1u;
+        const uint32_t base = __builtin_amdgcn_readlane(pv, (int)shd);
+        const uint32_t g = a.list[((size_t)cur * kShards + shd) * a.shard_cap + (j - base)];
+        if (tid == 0) claim_tile(a, g, p);
+        sweep_tile<TW, TH, KIND, false>(a, g, wave, nxt, p, j, sh, s_u32, s_f32, in0);
+        __syncthreads();
     }
 }
 
@@ -1346,7 +1431,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
     }
@@ -1477,7 +1562,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
         for (uint32_t j = blockIdx.x; j < nj; j += G) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
         // ---- arrive ----
@@ -1653,7 +1738,7 @@ __device__ __forceinline__ void side_step3(int s, int& dx, int& dy, int& dz) {
 // improves by >= eps or the pass cap, predict the six neighbour faces, activate the
 // neighbours whose face would improve, commit the improved cells.
 template <int KIND>
-__device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t cur,
+__device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t nxt,
                              uint32_t p, float* t_s, float* c_s, uint32_t* s_u32, float* s_f32,
                              GhcmInst in_single) {
     constexpr int B = kT3, PB = kT3Box, PB2 = PB * PB, PP = PB2 * PB, P = B * B * B;
@@ -1786,7 +1871,7 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
         side_step3(tid, dx, dy, dz);
         const uint32_t nb = in.tile_base + (uint32_t)(tx + dx) +
                             in.tiles_x * ((uint32_t)(ty + dy) + in.tiles_y * (uint32_t)(tz + dz));
-        activate(a, nb, wave, cur, p, kMaskOpenAll, s_f32[tid]);
+        activate(a, nb, wave, nxt, p, kMaskOpenAll, s_f32[tid]);
     }
 
     // Commit improved cells (one writer per tile per wave), tile minimum.
@@ -1820,7 +1905,7 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
         stat_add(&a.tstat[g], 1u);
         stat_add(&a.tstat[(size_t)a.tiles + g], passes);
         stat_add(&a.tstat[2 * (size_t)a.tiles + g], s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11]);
-        if (unfinished) activate(a, g, wave, cur, p, kMaskOpenAll, tm);
+        if (unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
     }
 }
 
@@ -1850,10 +1935,10 @@ __global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8)))
     for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
         const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
         if (tid == 0) claim_tile(a, g, p);
-        solve3d_tile<KIND>(a, g, wave, cur, p, t_s, c_s, s_u32, s_f32, in0);
+        solve3d_tile<KIND>(a, g, wave, cur ^ 1u, p, t_s, c_s, s_u32, s_f32, in0);
         __syncthreads();
     }
-    if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur, p);
+    if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
 }
 
 // Seeds of 3-D solves: (x, y, z, instance) quadruples; T = 0 at the seed, and the tiles of
@@ -2059,6 +2144,47 @@ SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host, bool
     if (tw == 16 && th == 16) return pick_mode<16, 16>(kind, mode, host);
     if (tw == 32 && th == 8) return pick_mode<32, 8>(kind, mode, host);
     if (tw == 32 && th == 32) return pick_mode<32, 32>(kind, mode, host);
+    return {nullptr, nullptr};
+}
+
+template <int TW, int TH, int KIND, int WPE>
+static hipError_t launch_bf_t(const SolveArgs& a, uint32_t wave, uint32_t grid, hipStream_t s,
+                              hipEvent_t ev0, hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE>), dim3(grid), dim3(kSweepThreads),
+                              0, s, ev0, ev1, 0, a, wave);
+    else
+        hipLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE>), dim3(grid), dim3(kSweepThreads), 0,
+                           s, a, wave);
+    return hipGetLastError();
+}
+
+template <int TW, int TH, int KIND, int WPE>
+static int occ_bf_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep_bf<TW, TH, KIND, WPE>,
+                                                     kSweepThreads, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+template <int TW, int TH, int WPE>
+static BfKernel pick_bf(int kind) {
+    switch (kind) {
+        case 0: return {&launch_bf_t<TW, TH, 0, WPE>, &occ_bf_t<TW, TH, 0, WPE>};
+        case 1: return {&launch_bf_t<TW, TH, 1, WPE>, &occ_bf_t<TW, TH, 1, WPE>};
+        case 2: return {&launch_bf_t<TW, TH, 2, WPE>, &occ_bf_t<TW, TH, 2, WPE>};
+        case 3: return {&launch_bf_t<TW, TH, 3, WPE>, &occ_bf_t<TW, TH, 3, WPE>};
+    }
+    return {nullptr, nullptr};
+}
+
+BfKernel get_bf_kernel(int tw, int th, int kind, int mode, bool wide) {
+    if (mode != 2) return {nullptr, nullptr};
+    if (tw == 64 && th == 64) return wide ? pick_bf<64, 64, 4>(kind) : pick_bf<64, 64, 1>(kind);
+    if (tw == 64 && th == 32) return pick_bf<64, 32, 1>(kind);
+    if (tw == 64 && th == 16) return pick_bf<64, 16, 1>(kind);
+    if (tw == 32 && th == 32) return pick_bf<32, 32, 1>(kind);
     return {nullptr, nullptr};
 }
 

@@ -100,7 +100,11 @@ typedef struct ghcm_config {
                                        its grid barrier pays agent-scope fences),
                                        2 = chunks of waves replayed as a HIP graph,
                                        3 = one launch per wave: the tile-solve kernel's
-                                       last work-group runs the wave boundary (sweep) */
+                                       last work-group runs the wave boundary (sweep),
+                                       4 = boundary-free waves (sweep kernel): one
+                                       tile-solve launch per wave and no wave-boundary
+                                       kernel; every queued tile is solved in its wave
+                                       (no top-n cut), n sets the launch's work-groups */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */

@@ -62,7 +62,7 @@ def test_rings256_every_config(rings256, kernel, tile, sched):
 @pytest.mark.parametrize("tile", SWEEP_TILES)
 @pytest.mark.parametrize("sched,launch_mode", [(g.SCHED_DEVICE_TOPN, 0), (g.SCHED_DEVICE_TOPN, 1),
                                                (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
-                                               (g.SCHED_HOST_PQ, 0)])
+                                               (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_HOST_PQ, 0)])
 @pytest.mark.parametrize("cap", [0, 1])
 def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     cost, seeds, ref32, ref64 = rings256
@@ -101,7 +101,7 @@ def test_long_lists_carried_by_the_tile_launch():
     check(t, ref, r.seeds, r.cost, label="long lists systolic")
 
 
-@pytest.mark.parametrize("launch_mode", [0, 1, 3])
+@pytest.mark.parametrize("launch_mode", [0, 1, 3, 4])
 def test_long_first_list_in_every_launch_mode(launch_mode):
     """Every 32x32 tile of a 2048^2 field is a walled chamber (cost +INF on its last row and
     column) with a source at its centre: the first boundary selects 64 of 4,096 queued tiles,
@@ -156,7 +156,7 @@ def test_ragged_obstacles_multi_seed(rng, kernel, shape):
     ref = oracle.fmm_f32(cost, seeds)
     tiles = ((64, 16), (32, 32)) if kernel == g.KERNEL_SWEEP else ((8, 8), (32, 8))
     for tile in tiles:
-        for lm in ((0, 1) if kernel == g.KERNEL_SWEEP else (0,)):
+        for lm in ((0, 1, 4) if kernel == g.KERNEL_SWEEP else (0,)):
             print(f"case ragged {shape} kernel={kernel} tile={tile} launch_mode={lm}", flush=True)
             with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=7,
                            launch_mode=lm) as ctx:

@@ -22,7 +22,7 @@ def test_plain4096_full_size_parity(plain4096):
     ref = oracle.fmm_f32(cost, seeds)
     for tile, kernel, lm in (((16, 16), g.KERNEL_SYSTOLIC, 0), ((8, 8), g.KERNEL_FIM, 0),
                              ((64, 64), g.KERNEL_SWEEP, 0), ((64, 64), g.KERNEL_SWEEP, 1),
-                             ((64, 64), g.KERNEL_SWEEP, 3)):
+                             ((64, 64), g.KERNEL_SWEEP, 3), ((64, 64), g.KERNEL_SWEEP, 4)):
         print(f"case plain4096 tile={tile} kernel={kernel} launch_mode={lm}", flush=True)
         with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, n_active=512,
                        inner_iter_cap=1 if kernel == g.KERNEL_SWEEP else 0, launch_mode=lm) as ctx:
