@@ -91,6 +91,8 @@ BATCH_CANDIDATES = [  # autotuner search space for the batch (kept small so the 
     dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=0),
     dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
     dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=2, launch_mode=2),
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=2048, inner_iter_cap=1, launch_mode=1),
+    dict(tile_w=64, tile_h=64, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=4),
     dict(tile_w=64, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
     dict(tile_w=32, tile_h=32, kernel=2, n_active=65536, inner_iter_cap=1, launch_mode=2),
     dict(tile_w=16, tile_h=16, kernel=0, n_active=65536, inner_iter_cap=0, launch_mode=2),

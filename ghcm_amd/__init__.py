@@ -115,6 +115,21 @@ class ServeStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TileDesc(ctypes.Structure):
+    _fields_ = [("tiles_x", ctypes.c_uint32), ("tiles_y", ctypes.c_uint32),
+                ("tx", ctypes.c_uint32), ("ty", ctypes.c_uint32),
+                ("x0", ctypes.c_uint32), ("y0", ctypes.c_uint32),
+                ("w", ctypes.c_uint32), ("h", ctypes.c_uint32),
+                ("neighbour", ctypes.c_uint32 * 4), ("border_bit", ctypes.c_uint32 * 4),
+                ("seed_tile", ctypes.c_uint32)]
+
+    def to_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["neighbour"] = list(self.neighbour)
+        d["border_bit"] = list(self.border_bit)
+        return d
+
+
 class Image(ctypes.Structure):
     _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("comp", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("data", ctypes.POINTER(ctypes.c_float))]
@@ -137,6 +152,7 @@ EXPORTED_SYMBOLS = (
     "ghcm_field_load_mono", "ghcm_field_load_rgb_luminosity", "ghcm_rgb_luminosity_device",
     "ghcm_write_time_field_png", "ghcm_write_field_hdr", "ghcm_blob_write", "ghcm_blob_read",
     "ghcm_test_run", "ghcm_field_cost_sin_checkerboard", "ghcm_field_cost_random",
+    "ghcm_tile_info", "ghcm_debug_select",
 )
 
 _lib = None
@@ -224,6 +240,9 @@ def lib() -> ctypes.CDLL:
                                 vp, u32, ctypes.c_char_p, u32, ctypes.c_float]
     L.ghcm_field_cost_sin_checkerboard.argtypes = [u32, u32, ctypes.c_float, ctypes.c_float, vp]
     L.ghcm_field_cost_random.argtypes = [u32, u32, ctypes.c_uint64, vp]
+    L.ghcm_tile_info.argtypes = [u32, u32, u32, u32, u32, u32, P(TileDesc)]
+    L.ghcm_debug_select.argtypes = [vp, u32, vp, vp, vp, u32, u32, i32, vp, P(u32), vp, P(u32),
+                                    vp, vp, vp, P(u32)]
     for name in EXPORTED_SYMBOLS:
         getattr(L, name).restype = getattr(L, name).restype or i32
     _lib = L
@@ -451,6 +470,34 @@ class Context:
         _check(self._L.ghcm_ctx_get_config(self._h, ctypes.byref(c)))
         self.config = c
         return cands[best.value].to_dict(), [x.to_dict() for x in cands]
+
+
+    def debug_select(self, tiles_total: int, tiles, keys, masks, n: int, handoff: bool = False):
+        """One device wave boundary on a given list (test support, ghcm_debug_select):
+        returns (jobs, handed, next_tiles, next_keys, next_masks)."""
+        t = np.ascontiguousarray(np.asarray(tiles, dtype=np.uint32))
+        k = np.ascontiguousarray(np.asarray(keys, dtype=np.float32))
+        m = np.ascontiguousarray(np.asarray(masks, dtype=np.uint32))
+        c = t.shape[0]
+        assert k.shape == (c,) and m.shape == (c,)
+        cap = max(c, 1)
+        jobs, handed, nxt = (np.zeros(cap, np.uint32) for _ in range(3))
+        nkeys, nmasks = np.zeros(cap, np.float32), np.zeros(cap, np.uint32)
+        nj, nh, nn = ctypes.c_uint32(0), ctypes.c_uint32(0), ctypes.c_uint32(0)
+        _check(self._L.ghcm_debug_select(self._h, tiles_total, t.ctypes.data, k.ctypes.data,
+                                         m.ctypes.data, c, n, 1 if handoff else 0,
+                                         jobs.ctypes.data, ctypes.byref(nj), handed.ctypes.data,
+                                         ctypes.byref(nh), nxt.ctypes.data, nkeys.ctypes.data,
+                                         nmasks.ctypes.data, ctypes.byref(nn)))
+        return (jobs[:nj.value], handed[:nh.value], nxt[:nn.value], nkeys[:nn.value],
+                nmasks[:nn.value])
+
+
+def tile_info(w: int, h: int, tile_w: int, tile_h: int, tile: int, tile_base: int = 0) -> dict:
+    """Geometry of one tile as the scheduler sees it (ghcm_tile_info, host only)."""
+    d = TileDesc()
+    _check(lib().ghcm_tile_info(w, h, tile_w, tile_h, tile_base, tile, ctypes.byref(d)))
+    return d.to_dict()
 
 
 class Server:

@@ -55,6 +55,15 @@ extern "C" int ghcm_tune_default_space(int cus, ghcm_tune_candidate* out, uint32
                 put(out, cap, &n, t[0], t[1], GHCM_KERNEL_FIM, na, 0u, mode);
             }
         }
+    // The sweep kernel's other wave loops: one persistent cooperative launch (1) and
+    // boundary-free waves (4, every queued tile each wave, n sets the grid).
+    for (uint32_t mode : {1u, 4u})
+        for (uint32_t m : mult) {
+            const uint32_t na = m * (uint32_t)cus;
+            for (const auto& t : large)
+                for (uint32_t inner : {1u, 0u})
+                    put(out, cap, &n, t[0], t[1], GHCM_KERNEL_SWEEP, na, inner, mode);
+        }
     *n_out = n < cap ? n : cap;
     if (n > cap) return fail(GHCM_ERR_INVALID_ARG, "tune space has %u candidates, capacity %u", n, cap);
     return GHCM_OK;

@@ -28,6 +28,7 @@
 #include "ghcm_dev.h"
 #include "ghcm_io.h"
 #include "kernels.h"
+#include "tile_geom.h"
 
 using namespace ghcm;
 
@@ -423,8 +424,8 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
         g.time = x.d_time;
         g.w = x.w;
         g.h = x.h;
-        g.tiles_x = (x.w + tw - 1) / tw;
-        g.tiles_y = (x.h + th - 1) / th;
+        g.tiles_x = tiles_along(x.w, tw);
+        g.tiles_y = tiles_along(x.h, th);
         g.tile_base = (uint32_t)tiles64;
         g.d = dd;
         g.tiles_z = (dd + td - 1) / td;
@@ -464,7 +465,10 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 hs[k++] = i;
             }
     }
+    // jobs[]: the selected tiles plus the carried tiles a long selection hands to the tile
+    // launch, so one word per tile
     if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, T)))) return rc;
+    if (ctx->sel.bytes < 4ull * T) return fail(GHCM_ERR_INVALID_ARG, "selection buffer below %u words", T);
     // Grid of the device-mode solve kernel: at most one resident round of work-groups;
     // each work-group loops over list entries, so the grid never has to match the list.
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
@@ -740,12 +744,10 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                       inst_of.begin() + hinst[i].tile_base + hinst[i].tiles_x * hinst[i].tiles_y, i);
         auto nbr = [&](uint32_t id, int s, uint32_t* out) -> bool {
             const GhcmInst& g = hinst[inst_of[id]];
-            uint32_t lt = id - g.tile_base;
-            int x = (int)(lt % g.tiles_x) + (s == 0 ? -1 : s == 2 ? 1 : 0);
-            int y = (int)(lt / g.tiles_x) + (s == 1 ? -1 : s == 3 ? 1 : 0);
-            if (x < 0 || y < 0 || x >= (int)g.tiles_x || y >= (int)g.tiles_y) return false;
-            *out = g.tile_base + (uint32_t)y * g.tiles_x + (uint32_t)x;
-            return true;
+            int tx, ty;
+            tile_xy(id - g.tile_base, g.tiles_x, tx, ty);
+            *out = tile_neighbour(g.tile_base, g.tiles_x, g.tiles_y, tx, ty, s);
+            return *out != kNoTile;
         };
         // BlockWrapper key: min smallest_time of the tile and its 4 neighbours, frozen
         // at push (src/eikonal_hcm.cpp:47-80).
@@ -773,7 +775,8 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 for (int d = 0; d < 5; ++d) {
                     int px = (int)hi[i].seeds[2 * s] + dx[d], py = (int)hi[i].seeds[2 * s + 1] + dy[d];
                     if (px < 0 || py < 0 || px >= (int)hi[i].w || py >= (int)hi[i].h) continue;
-                    uint32_t t = hinst[i].tile_base + (uint32_t)(py / (int)th) * hinst[i].tiles_x + (uint32_t)(px / (int)tw);
+                    const uint32_t t = tile_of_cell(hinst[i].tile_base, hinst[i].tiles_x, (uint32_t)px,
+                                                    (uint32_t)py, tw, th);
                     border[t] |= kMaskOpenAll;
                     smallest[t] = 0.0f;
                     if (!dirty[t]) { dirty[t] = 1; pq.push({0.0f, t}); }
@@ -813,7 +816,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 for (int side = 0; side < 4; ++side) {
                     uint32_t nb;
                     if (!s.border[side] || !nbr(id, side, &nb)) continue;
-                    border[nb] |= 1u << (side ^ 2);
+                    border[nb] |= border_bit_across(side);
                     if (!dirty[nb]) { dirty[nb] = 1; pq.push({key_of(nb), nb}); }
                 }
                 if (s.requeue) {
@@ -919,6 +922,8 @@ extern "C" int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_k
 extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
                           const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
                           ghcm_stats* stats) {
+  // The staging copies allocate (the copy pool's queue): nothing may escape the C boundary.
+  return no_throw([&]() -> int {
     if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
     if (!field || !time_out) return fail(GHCM_ERR_INVALID_ARG, "null field or output");
     if (!valid_kind(field_kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", field_kind);
@@ -938,6 +943,7 @@ extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint
         stats->duration_total_us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+  });
 }
 
 extern "C" int ghcm_solve3d_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
@@ -951,6 +957,7 @@ extern "C" int ghcm_solve3d_device(ghcm_ctx* ctx, const void* d_field, int field
 extern "C" int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
                             uint32_t d, const uint32_t* seeds_xyz, uint32_t n_seeds, float* time_out,
                             ghcm_stats* stats) {
+  return no_throw([&]() -> int {
     if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
     if (!field || !time_out) return fail(GHCM_ERR_INVALID_ARG, "null field or output");
     if (!valid_kind(field_kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", field_kind);
@@ -970,6 +977,7 @@ extern "C" int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, ui
         stats->duration_total_us =
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+  });
 }
 
 extern "C" int ghcm_dev_alloc(ghcm_ctx* ctx, size_t bytes, void** out) {
@@ -1036,5 +1044,139 @@ extern "C" int ghcm_solve_batch(ghcm_ctx* ctx, int field_kind, const ghcm_instan
         for (uint32_t i = 0; i < n_inst; ++i)
             hi[i] = HostInst{inst[i].d_field, inst[i].d_time, inst[i].w, inst[i].h, inst[i].seeds_xy, inst[i].n_seeds};
         return solve_core(ctx, field_kind, hi.data(), n_inst, stats);
+    });
+}
+
+// ---------------------------------------------------------------------------------------
+// test support
+// ---------------------------------------------------------------------------------------
+
+extern "C" int ghcm_tile_info(uint32_t w, uint32_t h, uint32_t tile_w, uint32_t tile_h,
+                              uint32_t tile_base, uint32_t tile, ghcm_tile_desc* out) {
+    if (!out) return fail(GHCM_ERR_INVALID_ARG, "out is null");
+    if (!w || !h || !tile_w || !tile_h) return fail(GHCM_ERR_INVALID_ARG, "zero size");
+    const uint32_t tx_n = tiles_along(w, tile_w), ty_n = tiles_along(h, tile_h);
+    if ((uint64_t)tile_base + (uint64_t)tx_n * ty_n > 0xFFFFFFFFull)
+        return fail(GHCM_ERR_INVALID_ARG, "tile ids overflow");
+    if (tile < tile_base || tile - tile_base >= tx_n * ty_n)
+        return fail(GHCM_ERR_INVALID_ARG, "tile %u outside [%u, %u)", tile, tile_base, tile_base + tx_n * ty_n);
+    std::memset(out, 0, sizeof *out);
+    int tx, ty;
+    tile_xy(tile - tile_base, tx_n, tx, ty);
+    out->tiles_x = tx_n;
+    out->tiles_y = ty_n;
+    out->tx = (uint32_t)tx;
+    out->ty = (uint32_t)ty;
+    out->x0 = (uint32_t)tx * tile_w;
+    out->y0 = (uint32_t)ty * tile_h;
+    out->w = (uint32_t)tile_extent((int)w, (int)out->x0, (int)tile_w);
+    out->h = (uint32_t)tile_extent((int)h, (int)out->y0, (int)tile_h);
+    for (int s = 0; s < 4; ++s) {
+        out->neighbour[s] = tile_neighbour(tile_base, tx_n, ty_n, tx, ty, s);
+        out->border_bit[s] = border_bit_across(s);
+    }
+    out->seed_tile = tile_of_cell(tile_base, tx_n, out->x0 + out->w - 1, out->y0 + out->h - 1, tile_w, tile_h);
+    return GHCM_OK;
+}
+
+extern "C" int ghcm_debug_select(ghcm_ctx* ctx, uint32_t T, const uint32_t* tiles, const float* keys,
+                                 const uint32_t* masks, uint32_t count, uint32_t n, int handoff,
+                                 uint32_t* jobs, uint32_t* n_jobs, uint32_t* handed, uint32_t* n_handed,
+                                 uint32_t* next, float* next_keys, uint32_t* next_masks,
+                                 uint32_t* n_next) {
+    return no_throw([&]() -> int {
+        if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+        if (!T || count > T || !n) return fail(GHCM_ERR_INVALID_ARG, "need 0 < n, count <= tiles_total");
+        if (count && (!tiles || !keys || !masks || !jobs || !handed || !next || !next_keys || !next_masks))
+            return fail(GHCM_ERR_INVALID_ARG, "null buffer");
+        if (!n_jobs || !n_handed || !n_next) return fail(GHCM_ERR_INVALID_ARG, "null count");
+        std::vector<uint8_t> seen(T, 0);
+        for (uint32_t i = 0; i < count; ++i) {
+            if (tiles[i] >= T || seen[tiles[i]]) return fail(GHCM_ERR_INVALID_ARG, "tile %u out of range or repeated", tiles[i]);
+            seen[tiles[i]] = 1;
+            if (!(keys[i] >= 0.0f)) return fail(GHCM_ERR_INVALID_ARG, "key %u is negative or NaN", i);
+        }
+        HIP_TRY(hipSetDevice(ctx->ordinal));
+        hipStream_t st = ctx->stream;
+        const uint32_t shard_cap = (T + kShards - 1) / kShards;
+        int rc;
+        if ((rc = ctx->qwave.reserve(4ull * T))) return rc;
+        if ((rc = ctx->key.reserve(8ull * T))) return rc;
+        if ((rc = ctx->mask.reserve(8ull * T))) return rc;
+        if ((rc = ctx->list.reserve(12ull * kShards * shard_cap))) return rc;
+        if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
+        if ((rc = ctx->sel.reserve(4ull * T))) return rc;
+        // Wave W's list is list 0.  The first boundary selects from it as it stands; a later
+        // boundary first swaps lists (it empties list 1, the one its wave consumed).
+        const uint32_t W = handoff ? 1u : 0u, p = W & 1u;
+        std::vector<uint32_t> hkey(2ull * T, kInfBits), hmask(2ull * T, 0u), hq(T, 0u);
+        std::vector<uint32_t> hlist(3ull * kShards * shard_cap, 0u);
+        std::vector<uint32_t> fill(kShards, 0u);
+        GhcmCtrl hc;
+        std::memset(&hc, 0, sizeof hc);
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint32_t g = tiles[i], sh = g % kShards;
+            hlist[(size_t)sh * shard_cap + fill[sh]++] = g;
+            uint32_t kb;
+            std::memcpy(&kb, &keys[i], 4);
+            hkey[(size_t)p * T + g] = kb;
+            hmask[(size_t)p * T + g] = masks[i];
+            hq[g] = W + 1u;
+        }
+        for (uint32_t sh = 0; sh < kShards; ++sh) hc.count[0][sh * kCounterStride] = fill[sh];
+        hc.cur = handoff ? 1u : 0u;
+        hc.wave = 0u;
+        hc.n_select = n;
+        HIP_TRY(hipMemcpyAsync(ctx->ctrl.p, &hc, sizeof hc, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->key.p, hkey.data(), 8ull * T, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->mask.p, hmask.data(), 8ull * T, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->qwave.p, hq.data(), 4ull * T, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(ctx->list.p, hlist.data(), hlist.size() * 4, hipMemcpyHostToDevice, st));
+        SolveArgs a{};
+        a.qwave = ctx->qwave.as<uint32_t>();
+        a.key = ctx->key.as<uint32_t>();
+        a.mask = ctx->mask.as<uint32_t>();
+        a.list = ctx->list.as<uint32_t>();
+        a.tiles = T;
+        a.shard_cap = shard_cap;
+        a.jobs = ctx->sel.as<uint32_t>();
+        a.ctrl = ctx->ctrl.as<GhcmCtrl>();
+        HIP_TRY(launch_wave_end(a, handoff ? 0u : 1u, st));
+        std::vector<uint32_t> hsel(T);
+        HIP_TRY(hipMemcpyAsync(&hc, ctx->ctrl.p, sizeof hc, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hsel.data(), ctx->sel.p, 4ull * T, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hlist.data(), ctx->list.p, hlist.size() * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hkey.data(), ctx->key.p, 8ull * T, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hmask.data(), ctx->mask.p, 8ull * T, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const uint32_t cur = hc.cur;          // the list the boundary selected from
+        if (hc.n_jobs > count || hc.n_carry > count - std::min(count, hc.n_jobs))
+            return fail(GHCM_ERR_HIP, "selection counts out of range (%u jobs, %u carried)", hc.n_jobs, hc.n_carry);
+        *n_jobs = hc.n_jobs;
+        if (hc.jobs_from_list) {
+            uint32_t k = 0;
+            for (uint32_t sh = 0; sh < kShards && k < hc.n_jobs; ++sh)
+                for (uint32_t i = 0; i < hc.count[cur][sh * kCounterStride] && k < hc.n_jobs; ++i)
+                    jobs[k++] = hlist[((size_t)cur * kShards + sh) * shard_cap + i];
+        } else {
+            for (uint32_t k = 0; k < hc.n_jobs; ++k) jobs[k] = hsel[k];
+        }
+        *n_handed = hc.n_carry;
+        for (uint32_t k = 0; k < hc.n_carry; ++k) handed[k] = hsel[hc.n_jobs + k];
+        const uint32_t nl = cur ^ 1u, np = (W + 1u) & 1u;
+        uint32_t m = 0;
+        for (uint32_t sh = 0; sh < kShards; ++sh) {
+            const uint32_t c = hc.count[nl][sh * kCounterStride];
+            if (m + c > count || c > shard_cap) return fail(GHCM_ERR_HIP, "next list longer than the input");
+            for (uint32_t i = 0; i < c; ++i, ++m) {
+                const uint32_t g = hlist[((size_t)nl * kShards + sh) * shard_cap + i];
+                if (g >= T) return fail(GHCM_ERR_HIP, "next list entry %u out of range", g);
+                next[m] = g;
+                std::memcpy(&next_keys[m], &hkey[(size_t)np * T + g], 4);
+                next_masks[m] = hmask[(size_t)np * T + g];
+            }
+        }
+        *n_next = m;
+        return (int)GHCM_OK;
     });
 }

@@ -35,6 +35,7 @@
 
 #include "ghcm_dev.h"
 #include "kernels.h"
+#include "tile_geom.h"
 
 namespace ghcm {
 
@@ -290,12 +291,11 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
 
     const uint32_t ii = (a.n_inst == 1) ? 0u : a.tinst[g];
     const GhcmInst in = uniform_inst(a.inst[ii]);
-    const uint32_t lt = g - in.tile_base;
-    const int tx = (int)(lt % in.tiles_x);
-    const int ty = (int)(lt / in.tiles_x);
+    int tx, ty;
+    tile_xy(g - in.tile_base, in.tiles_x, tx, ty);
     const int ox = tx * TW, oy = ty * TH;
     const int W = (int)in.w, H = (int)in.h;
-    const int sx = min(TW, W - ox), sy = min(TH, H - oy);
+    const int sx = tile_extent(W, ox, TW), sy = tile_extent(H, oy, TH);
 
     // ---- stage tile + halo ring into LDS (out of grid -> +INF) ----
     stage_tile<PW, PH, 64, KIND>(in, ox, oy, lane, t_s, c_s);
@@ -522,10 +522,8 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
         }
         if (lane < 4 && ((side_flags >> lane) & 1u)) {
             const int s = lane;
-            int nx = tx + (s == 0 ? -1 : s == 2 ? 1 : 0);
-            int ny = ty + (s == 1 ? -1 : s == 3 ? 1 : 0);
-            uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
-            activate(a, nb, wave, nxt, p, 1u << (s ^ 2), side_key[s]);
+            const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, s);
+            activate(a, nb, wave, nxt, p, border_bit_across(s), side_key[s]);
         }
     }
 }
@@ -725,12 +723,11 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     // addresses would keep it in scratch behind flat loads.
     GhcmInst in = in_single;
     if (a.n_inst != 1) in = uniform_inst(a.inst[a.tinst[g]]);
-    const uint32_t lt = g - in.tile_base;
-    const int tx = (int)(lt % in.tiles_x);
-    const int ty = (int)(lt / in.tiles_x);
+    int tx, ty;
+    tile_xy(g - in.tile_base, in.tiles_x, tx, ty);
     const int ox = tx * TW, oy = ty * TH;
     const int W = (int)in.w, H = (int)in.h;
-    const int sx = min(TW, W - ox), sy = min(TH, H - oy);
+    const int sx = tile_extent(W, ox, TW), sy = tile_extent(H, oy, TH);
 
     stage_tile<PW, PH, kSweepThreads, KIND>(in, ox, oy, tid, t_s, c_s);
     __syncthreads();
@@ -848,10 +845,8 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         __syncthreads();
         if (tid < 4 && s_u32[4 + tid]) {
             const int sd = tid;
-            const int nx = tx + (sd == 0 ? -1 : sd == 2 ? 1 : 0);
-            const int ny = ty + (sd == 1 ? -1 : sd == 3 ? 1 : 0);
-            const uint32_t nb = in.tile_base + (uint32_t)ny * in.tiles_x + (uint32_t)nx;
-            activate(a, nb, wave, nxt, p, 1u << (sd ^ 2), s_f32[sd]);
+            const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, sd);
+            activate(a, nb, wave, nxt, p, border_bit_across(sd), s_f32[sd]);
         }
     }
 
@@ -1424,7 +1419,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
     const uint32_t wave = ctrl->wave;
     const uint32_t cur = ctrl->cur;
     const uint32_t from_list = ctrl->jobs_from_list;
-    if (!done && blockIdx.x < nj) {
+    // Tiles a wave boundary handed to this launch (none unless a k_wave_end after a solve's
+    // first boundary selected from a long list) are carried like in k_solve_sweep.
+    const uint32_t ncarry = ctrl->n_carry;
+    if (!done && (blockIdx.x < nj || ncarry != 0u)) {
         const uint32_t p = wave & 1u;
         for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
         __syncthreads();
@@ -1434,6 +1432,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
             sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1479,7 +1478,7 @@ __global__ void k_seed_serve(SolveArgs a, const uint32_t* seeds, uint32_t n_seed
     for (int d = 0; d < 5; ++d) {
         const int px = (int)x + dx[d], py = (int)y + dy[d];
         if (px < 0 || py < 0 || px >= (int)in.w || py >= (int)in.h) continue;
-        const uint32_t t = in.tile_base + (uint32_t)(py / (int)th) * in.tiles_x + (uint32_t)(px / (int)tw);
+        const uint32_t t = tile_of_cell(in.tile_base, in.tiles_x, (uint32_t)px, (uint32_t)py, tw, th);
         atomicMin(&a.key[(size_t)np * a.tiles + t], 0u);
         atomicOr(&a.mask[(size_t)np * a.tiles + t], kMaskOpenAll);
         const uint32_t old = atomicMax(&a.qwave[t], wave + 2u);
@@ -1556,6 +1555,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
         const uint32_t cur = ctrl->cur;
         const uint32_t nj = ctrl->n_jobs;
         const uint32_t from_list = ctrl->jobs_from_list;
+        const uint32_t ncarry = ctrl->n_carry;
         const uint32_t p = wave & 1u;
         for (int i = tid; i <= (int)kShards; i += kSweepThreads) s_prefix[i] = ctrl->prefix[i];
         __syncthreads();
@@ -1565,6 +1565,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
             sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
+        // Tiles a boundary handed to the tile loop (none from this kernel's own selections,
+        // which carry directly) are carried by the whole grid.
+        if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
         // ---- arrive ----
         __threadfence();
         __syncthreads();
@@ -1653,7 +1656,7 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
     for (int d = 0; d < 5; ++d) {
         int px = (int)x + dx[d], py = (int)y + dy[d];
         if (px < 0 || py < 0 || px >= (int)in.w || py >= (int)in.h) continue;
-        uint32_t t = in.tile_base + (uint32_t)(py / (int)th) * in.tiles_x + (uint32_t)(px / (int)tw);
+        const uint32_t t = tile_of_cell(in.tile_base, in.tiles_x, (uint32_t)px, (uint32_t)py, tw, th);
         atomicMin(&a.key[t], 0u);
         atomicOr(&a.mask[t], kMaskOpenAll);
         uint32_t old = atomicMax(&a.qwave[t], 1u);

@@ -383,6 +383,14 @@ bool png_decode(const uint8_t* data, size_t n, PngImage& img, std::string& err) 
         const size_t ph = h > (uint32_t)py0 ? (h - py0 + pdy - 1) / pdy : 0;
         if (pw && ph) total += ((pw * bits_pp + 7) / 8 + 1) * ph;
     }
+    // Decoded samples are held as 16-bit words: an image above 2^31 samples (4 GiB of
+    // samples, whatever its bit depth) is refused before anything is inflated, so a small file
+    // with a huge header and a highly compressible stream cannot ask for tens of GB.
+    constexpr uint64_t kMaxSamples = uint64_t(1) << 31;
+    if ((uint64_t)w * h * (uint64_t)channels > kMaxSamples) {
+        err = "image too large (more than 2^31 samples)";
+        return false;
+    }
     std::vector<uint8_t> raw;
     if (!zlib_inflate(idat.data(), idat.size(), raw, err, total)) return false;
     if (raw.size() < total) { err = "image data too short"; return false; }

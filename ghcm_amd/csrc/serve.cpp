@@ -47,6 +47,7 @@ using ghcm_capi::PinnedBuf;
 using ghcm_capi::par_memcpy;
 using ghcm_capi::fail;
 using ghcm_capi::kind_bytes;
+using ghcm_capi::no_throw;
 
 namespace {
 
@@ -784,6 +785,8 @@ int enqueue_request(ghcm_server* s, Request* r, uint64_t* ticket) {
 extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, uint32_t h,
                                  const uint32_t* seeds_xy, uint32_t n_seeds, float* time_out,
                                  uint64_t* ticket) {
+  // The staging copy allocates (the copy pool's queue): nothing may escape the C boundary.
+  return no_throw([&]() -> int {
     if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
     if (!field || !time_out || !ticket) return fail(GHCM_ERR_INVALID_ARG, "null argument");
     // The pinned staging below is mapped for, and addressed on, the server's device.
@@ -801,15 +804,24 @@ extern "C" int ghcm_serve_submit(ghcm_server* s, const void* field, uint32_t w, 
     const size_t fbytes = (size_t)w * h * kind_bytes(s->kind), tbytes = (size_t)w * h * 4;
     r->h_field = s->pool.get(fbytes, &r->h_field_dev);
     r->h_time = s->pool.get(tbytes, &r->h_time_dev);
-    if (!r->h_field || !r->h_time) {
+    auto drop = [&] {
         s->pool.put(r->h_field, fbytes);
         s->pool.put(r->h_time, tbytes);
         delete r;
+    };
+    if (!r->h_field || !r->h_time) {
+        drop();
         return fail(GHCM_ERR_OUT_OF_MEMORY, "pinned staging of %zu bytes failed", fbytes + tbytes);
     }
-    par_memcpy(r->h_field, field, fbytes);
+    try {
+        par_memcpy(r->h_field, field, fbytes);
+    } catch (...) {
+        drop();
+        throw;
+    }
     r->time_out = time_out;
     return enqueue_request(s, r, ticket);
+  });
 }
 
 extern "C" int ghcm_serve_submit_device(ghcm_server* s, const void* d_field, uint32_t w,
@@ -832,6 +844,7 @@ extern "C" int ghcm_serve_submit_device(ghcm_server* s, const void* d_field, uin
 
 extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s,
                                ghcm_serve_result* out) {
+  return no_throw([&]() -> int {
     if (!s) return fail(GHCM_ERR_INVALID_ARG, "server is null");
     std::unique_lock<std::mutex> lk(s->mu);
     // ghcm_serve_stop frees the server only once no wait is inside this function.
@@ -862,10 +875,17 @@ extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s
     s->tickets.erase(ticket);
     const std::string fatal_msg = s->fatal_msg;
     lk.unlock();
-    const int status = r->status;
-    // Host request: the result lands in the caller's buffer from the pinned staging.
-    if (!r->device && status == GHCM_OK)
-        par_memcpy(r->time_out, r->h_time, (size_t)r->w * r->h * 4);
+    int status = r->status;
+    // Host request: the result lands in the caller's buffer from the pinned staging.  The
+    // request is collected (its ticket erased) whatever happens here, so a failed copy is
+    // reported and the request still freed below.
+    if (!r->device && status == GHCM_OK) {
+        const int rc_copy = no_throw([&] {
+            par_memcpy(r->time_out, r->h_time, (size_t)r->w * r->h * 4);
+            return (int)GHCM_OK;
+        });
+        if (rc_copy != GHCM_OK) status = rc_copy;
+    }
     const auto t_out = Clock::now();
     if (out) {
         std::memset(out, 0, sizeof *out);
@@ -885,6 +905,7 @@ extern "C" int ghcm_serve_wait(ghcm_server* s, uint64_t ticket, double timeout_s
         return fail(status, "request %llu: %s", (unsigned long long)ticket,
                     fatal_msg.empty() ? ghcm_status_string(status) : fatal_msg.c_str());
     return GHCM_OK;
+  });
 }
 
 extern "C" int ghcm_serve_get_stats(ghcm_server* s, ghcm_serve_stats* out) {

@@ -323,6 +323,45 @@ int ghcm_gen_smooth_speed(uint32_t w, uint32_t h, uint64_t seed, double sigma,
                           double speed_lo, double speed_hi, double* speed_out);
 int ghcm_gen_points(uint32_t w, uint32_t h, uint64_t seed, uint32_t n, uint32_t* xy_out);
 
+/* ---- test support: the scheduler's bookkeeping, callable on its own -----------
+ * Neither function has a counterpart in the reference; they expose the library's own
+ * tile bookkeeping (the block grid and border masks of src/eikonal_hcm.cpp:203-224 and
+ * :442-469, and the wave pop of :344-358) so tests can pin it directly. */
+
+/* Geometry of global tile `tile` of a w x h instance in tile_w x tile_h tiles whose ids
+ * start at tile_base: row-major ids, partial edge tiles, the neighbour across each side
+ * (0 x-, 1 y-, 2 x+, 3 y+; 0xFFFFFFFF at the border) and the border bit that neighbour
+ * receives when this tile's edge on that side changes.  Host only (no device needed). */
+typedef struct ghcm_tile_desc {
+    uint32_t tiles_x, tiles_y;      /* tiles of the instance                          */
+    uint32_t tx, ty;                /* this tile's coordinates                        */
+    uint32_t x0, y0, w, h;          /* its cells: origin and extent                   */
+    uint32_t neighbour[4];          /* global id across side s, or 0xFFFFFFFF         */
+    uint32_t border_bit[4];         /* bit set on neighbour[s] by a change on side s  */
+    uint32_t seed_tile;             /* tile holding cell (x0 + w - 1, y0 + h - 1)     */
+} ghcm_tile_desc;
+int ghcm_tile_info(uint32_t w, uint32_t h, uint32_t tile_w, uint32_t tile_h, uint32_t tile_base,
+                   uint32_t tile, ghcm_tile_desc* out);
+
+/* One device wave boundary (the top-n selection of k_wave_end) run on a given list:
+ * `count` distinct tiles of a `tiles_total`-tile grid with their keys (the smallest
+ * candidate time that queued them, >= 0 or +INF) and border masks, queued for wave 0 in
+ * list order over the 32 shards (tile g goes to shard g % 32).  n: tiles per wave.
+ * handoff = 0 runs the solve's first boundary (unselected tiles are carried to the next
+ * list by the boundary itself); handoff = 1 runs a later boundary, which leaves the
+ * unselected tiles of a long list (more than 2,048 entries) to the next tile launch.
+ * Outputs (capacity `count` each):
+ *   jobs[0, *n_jobs)            the selected tiles, in list order;
+ *   handed[0, *n_handed)        tiles left to the tile launch (handoff only), list order;
+ *   next[0, *n_next)            the next wave's list as the boundary left it, shard by
+ *                               shard, with next_keys / next_masks of each entry.
+ * A test entry point: it overwrites the context's scheduler buffers. */
+int ghcm_debug_select(ghcm_ctx* ctx, uint32_t tiles_total, const uint32_t* tiles,
+                      const float* keys, const uint32_t* masks, uint32_t count, uint32_t n,
+                      int handoff, uint32_t* jobs, uint32_t* n_jobs, uint32_t* handed,
+                      uint32_t* n_handed, uint32_t* next, float* next_keys,
+                      uint32_t* next_masks, uint32_t* n_next);
+
 #ifdef __cplusplus
 }
 #endif

@@ -302,10 +302,23 @@ def _ihdr_png(w, h, idat: bytes) -> bytes:
 
 
 def test_png_with_huge_header_and_tiny_data_is_refused():
-    """A 16M x 16M grey header over a few bytes of data: an error, not an allocation."""
-    png = _ihdr_png(1 << 24, 1 << 24, zlib.compress(bytes(64)))
+    """A 40,000 x 40,000 grey header (1.6e9 samples, under the sample cap) over a few bytes of
+    data: an error, not an allocation."""
+    png = _ihdr_png(40000, 40000, zlib.compress(bytes(64)))
     with pytest.raises(g.GhcmError, match="too short"):
         g.png_decode8(png)
+
+
+def test_png_above_the_sample_cap_is_refused_before_inflating():
+    """A 16M x 16M grey header (2^48 samples) over a stream that would inflate to 64 MiB:
+    refused from the header alone (more than 2^31 samples), before any inflation."""
+    png = _ihdr_png(1 << 24, 1 << 24, zlib.compress(bytes(64 << 20), 9))
+    assert len(png) < 200_000
+    with pytest.raises(g.GhcmError, match="too large"):
+        g.png_decode8(png)
+    # just above the cap: 46,341^2 grey samples > 2^31
+    with pytest.raises(g.GhcmError, match="too large"):
+        g.png_decode8(_ihdr_png(46341, 46341, zlib.compress(bytes(64))))
 
 
 def test_png_inflation_stops_at_the_image_size():
