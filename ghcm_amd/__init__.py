@@ -152,7 +152,7 @@ EXPORTED_SYMBOLS = (
     "ghcm_field_load_mono", "ghcm_field_load_rgb_luminosity", "ghcm_rgb_luminosity_device",
     "ghcm_write_time_field_png", "ghcm_write_field_hdr", "ghcm_blob_write", "ghcm_blob_read",
     "ghcm_test_run", "ghcm_field_cost_sin_checkerboard", "ghcm_field_cost_random",
-    "ghcm_tile_info", "ghcm_debug_select",
+    "ghcm_tile_info", "ghcm_debug_select", "ghcm_solve_device_noalloc",
 )
 
 _lib = None
@@ -195,6 +195,7 @@ def lib() -> ctypes.CDLL:
     L.ghcm_ctx_set_config.argtypes = [vp, P(Config)]
     L.ghcm_solve.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_device.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
+    L.ghcm_solve_device_noalloc.argtypes = [vp, vp, i32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve3d.argtypes = [vp, vp, i32, u32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve3d_device.argtypes = [vp, vp, i32, u32, u32, u32, vp, u32, vp, P(Stats)]
     L.ghcm_solve_batch.argtypes = [vp, i32, P(Instance), u32, P(Stats)]
@@ -415,6 +416,16 @@ class Context:
         _check(self._L.ghcm_solve_device(self._h, ctypes.c_void_p(d_field), kind, w, h,
                                          s.ctypes.data, s.shape[0], ctypes.c_void_p(d_time),
                                          ctypes.byref(st)))
+        return st.to_dict()
+
+    def solve_device_noalloc(self, d_field: int, kind: int, w: int, h: int, seeds,
+                             d_time: int) -> dict:
+        """ghcm_solve_device_noalloc: the same solve with the scheduler state in the device's
+        static scratch array (no allocation in the call)."""
+        s = _seeds_array(seeds)
+        st = Stats()
+        _check(self._L.ghcm_solve_device_noalloc(self._h, d_field, kind, w, h, s.ctypes.data,
+                                                 s.shape[0], d_time, ctypes.byref(st)))
         return st.to_dict()
 
     @staticmethod

@@ -45,6 +45,9 @@ using ghcm_capi::valid_kind;
 
 namespace {
 thread_local std::string g_last_error;
+// Seeds the allocation-free solve accepts (its pinned seed list is sized when a context is
+// created).
+constexpr uint32_t kNoAllocSeeds = 4096;
 }  // namespace
 
 int ghcm_capi::fail(int status, const char* fmt, ...) {
@@ -303,6 +306,22 @@ extern "C" int ghcm_ctx_create(int ordinal, const ghcm_config* cfg, ghcm_ctx** o
         return fail(GHCM_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
     }
     ctx->cus = p.multiProcessorCount;
+    // Set-up for the allocation-free solve, done here so that entry point allocates nothing:
+    // this device's address of the static scratch array (looked up once per device, with the
+    // device current), and the pinned records every solve uses -- the progress record, one
+    // instance entry and room for kNoAllocSeeds seeds.  They only grow later.
+    e = noalloc_scratch_init(ordinal);
+    if (e != hipSuccess) {
+        ghcm_ctx_destroy(ctx);
+        return fail(GHCM_ERR_HIP, "scratch address lookup failed: %s", hipGetErrorString(e));
+    }
+    int rc2 = ctx->rec.reserve(sizeof(HostRec));
+    if (!rc2) rc2 = ctx->h_inst.reserve(sizeof(GhcmInst));
+    if (!rc2) rc2 = ctx->h_seeds.reserve(3ull * 4 * kNoAllocSeeds);
+    if (rc2) {
+        ghcm_ctx_destroy(ctx);
+        return rc2;
+    }
     *out = ctx;
     return GHCM_OK;
 }
@@ -367,20 +386,75 @@ struct HostInst {      // host view of one instance
 // One device wave loop over n_inst instances.  three_d: every instance is a W x H x D grid
 // solved in 8x8x8 tiles by the 3-D tile kernel (device scheduler, per-wave launches or
 // graph replay); the configuration's tile shape and in-tile style do not apply.
+// noalloc: the allocation-free variant (ghcm_solve_device_noalloc): every device buffer is
+// carved out of the device's static scratch array, the pinned host records must already
+// hold what the solve needs (ghcm_ctx_create reserves them), and configurations that would
+// allocate (graph capture, per-launch profiling events, the host scheduler's containers,
+// the persistent and fused loops' extra records) are refused.
 int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
-                    bool three_d);
+                    bool three_d, bool noalloc);
 
 int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
-               bool three_d = false) {
-    return no_throw([&] { return solve_core_body(ctx, kind, hi, n_inst, stats, three_d); });
+               bool three_d = false, bool noalloc = false) {
+    return no_throw([&] { return solve_core_body(ctx, kind, hi, n_inst, stats, three_d, noalloc); });
+}
+
+// Byte offsets of the scheduler state of a T-tile solve inside the scratch array; each
+// region starts on a 256-byte boundary; `end` is the bytes used.
+struct ScratchLayout {
+    size_t ctrl, inst, totals, qwave, key, mask, tmin, tstat, sel, list, end;
+};
+ScratchLayout scratch_layout(uint32_t T, uint32_t shard_cap) {
+    auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+    ScratchLayout l{};
+    size_t o = 0;
+    l.ctrl = o;   o = up(o + sizeof(GhcmCtrl));
+    l.inst = o;   o = up(o + sizeof(GhcmInst));
+    l.totals = o; o = up(o + 3 * sizeof(unsigned long long));
+    l.qwave = o;  o = up(o + 4ull * T);
+    l.key = o;    o = up(o + 8ull * T);
+    l.mask = o;   o = up(o + 8ull * T);
+    l.tmin = o;   o = up(o + 4ull * T);
+    l.tstat = o;  o = up(o + 12ull * T);
+    l.sel = o;    o = up(o + 4ull * std::max<uint32_t>(1, T));
+    l.list = o;   o = up(o + 12ull * kShards * shard_cap);
+    l.end = o;
+    return l;
+}
+
+// Largest tile count whose scheduler state fits the scratch array.
+uint32_t noalloc_max_tiles() {
+    uint32_t lo = 1, hi = 1u << 24;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (scratch_layout(mid, (mid + kShards - 1) / kShards).end <= kNoAllocScratchBytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghcm_stats* stats,
-                    bool three_d) {
+                    bool three_d, bool noalloc) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     const ghcm_config& cfg = ctx->cfg;
     if (!valid_kind(kind)) return fail(GHCM_ERR_INVALID_ARG, "unknown field kind %d", kind);
     if (n_inst == 0) return fail(GHCM_ERR_INVALID_ARG, "no instances");
+    if (noalloc) {
+        if (n_inst != 1 || three_d) return fail(GHCM_ERR_INVALID_ARG, "allocation-free solve: one 2-D instance");
+        if (cfg.scheduler != GHCM_SCHED_DEVICE_TOPN || (cfg.launch_mode != 0 && cfg.launch_mode != 4) ||
+            cfg.profile_kernels)
+            return fail(GHCM_ERR_UNSUPPORTED, "allocation-free solve: device scheduler, launch_mode 0 or 4, "
+                                              "no kernel profiling");
+    }
+    // Pinned host records: grown on demand, or (allocation-free solve) checked against what
+    // the context reserved when it was created.
+    auto pinned = [noalloc](PinnedBuf& b, size_t want, const char* what) -> int {
+        if (!noalloc) return b.reserve(want);
+        if (b.bytes < want)
+            return fail(GHCM_ERR_INVALID_ARG, "allocation-free solve: %s needs %zu bytes, %zu reserved", what,
+                        want, b.bytes);
+        return GHCM_OK;
+    };
     const uint32_t tw = three_d ? kTile3D : cfg.tile_w, th = three_d ? kTile3D : cfg.tile_h;
     const uint32_t td = three_d ? kTile3D : 1u;
     const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
@@ -400,7 +474,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     const uint32_t sd = three_d ? 3u : 2u;            // seed components
 
     // ---- instance table, seeds ----
-    int rc = ctx->h_inst.reserve(sizeof(GhcmInst) * n_inst);
+    int rc = pinned(ctx->h_inst, sizeof(GhcmInst) * n_inst, "the instance table");
     if (rc) return rc;
     GhcmInst* hinst = ctx->h_inst.as<GhcmInst>();
     uint64_t tiles64 = 0, cells = 0, n_seeds_total = 0;
@@ -440,21 +514,64 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
 
     HIP_TRY(hipSetDevice(ctx->ordinal));
     hipStream_t st = ctx->stream;
-    if ((rc = ctx->inst.reserve(sizeof(GhcmInst) * n_inst))) return rc;
-    if ((rc = ctx->qwave.reserve(4ull * T))) return rc;
-    if ((rc = ctx->key.reserve(8ull * T))) return rc;
-    if ((rc = ctx->mask.reserve(8ull * T))) return rc;
-    if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
     const uint32_t shard_cap = (T + kShards - 1) / kShards;
-    // three lists: boundary-free waves rotate them (the other modes use two)
-    if ((rc = ctx->list.reserve(12ull * kShards * shard_cap))) return rc;
-    if ((rc = ctx->tstat.reserve(12ull * T))) return rc;
-    if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
-    if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
-    if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
-    if ((rc = ctx->h_seeds.reserve(4ull * (sd + 1) * std::max<uint64_t>(n_seeds_total, 1)))) return rc;
-    if ((rc = ctx->h_ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
-    if ((rc = ctx->rec.reserve(sizeof(HostRec)))) return rc;
+    // Device scheduler state: the context's grow-only buffers, or the scratch array.
+    GhcmInst* d_inst = nullptr;
+    uint32_t *d_tinst = nullptr, *d_qwave = nullptr, *d_key = nullptr, *d_mask = nullptr;
+    uint32_t *d_list = nullptr, *d_tstat = nullptr, *d_sel = nullptr;
+    float* d_tmin = nullptr;
+    unsigned long long* d_totals = nullptr;
+    GhcmCtrl* d_ctrl = nullptr;
+    if (noalloc) {
+        char* const base = static_cast<char*>(noalloc_scratch(ctx->ordinal));
+        if (!base) return fail(GHCM_ERR_HIP, "allocation-free solve: no scratch address for device %d", ctx->ordinal);
+        const ScratchLayout l = scratch_layout(T, shard_cap);
+        if (l.end > kNoAllocScratchBytes)
+            return fail(GHCM_ERR_INVALID_ARG, "allocation-free solve: %u tiles of %ux%u need %zu bytes of "
+                                              "scratch, above its %zu (at most %u tiles)", T, tw, th, l.end,
+                        kNoAllocScratchBytes, noalloc_max_tiles());
+        d_ctrl = reinterpret_cast<GhcmCtrl*>(base + l.ctrl);
+        d_inst = reinterpret_cast<GhcmInst*>(base + l.inst);
+        d_totals = reinterpret_cast<unsigned long long*>(base + l.totals);
+        d_qwave = reinterpret_cast<uint32_t*>(base + l.qwave);
+        d_key = reinterpret_cast<uint32_t*>(base + l.key);
+        d_mask = reinterpret_cast<uint32_t*>(base + l.mask);
+        d_tmin = reinterpret_cast<float*>(base + l.tmin);
+        d_tstat = reinterpret_cast<uint32_t*>(base + l.tstat);
+        d_sel = reinterpret_cast<uint32_t*>(base + l.sel);
+        d_list = reinterpret_cast<uint32_t*>(base + l.list);
+    } else {
+        if ((rc = ctx->inst.reserve(sizeof(GhcmInst) * n_inst))) return rc;
+        if ((rc = ctx->qwave.reserve(4ull * T))) return rc;
+        if ((rc = ctx->key.reserve(8ull * T))) return rc;
+        if ((rc = ctx->mask.reserve(8ull * T))) return rc;
+        if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
+        // three lists: boundary-free waves rotate them (the other modes use two)
+        if ((rc = ctx->list.reserve(12ull * kShards * shard_cap))) return rc;
+        if ((rc = ctx->tstat.reserve(12ull * T))) return rc;
+        if ((rc = ctx->totals.reserve(3 * sizeof(unsigned long long)))) return rc;
+        if ((rc = ctx->ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
+        if (n_inst > 1 && (rc = ctx->tinst.reserve(4ull * T))) return rc;
+        // jobs[]: the selected tiles plus the carried tiles a long selection hands to the tile
+        // launch, so one word per tile
+        if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, T)))) return rc;
+        if (ctx->sel.bytes < 4ull * T) return fail(GHCM_ERR_INVALID_ARG, "selection buffer below %u words", T);
+        d_inst = ctx->inst.as<GhcmInst>();
+        d_tinst = n_inst > 1 ? ctx->tinst.as<uint32_t>() : nullptr;
+        d_qwave = ctx->qwave.as<uint32_t>();
+        d_key = ctx->key.as<uint32_t>();
+        d_mask = ctx->mask.as<uint32_t>();
+        d_tmin = ctx->tmin.as<float>();
+        d_list = ctx->list.as<uint32_t>();
+        d_tstat = ctx->tstat.as<uint32_t>();
+        d_totals = ctx->totals.as<unsigned long long>();
+        d_ctrl = ctx->ctrl.as<GhcmCtrl>();
+        d_sel = ctx->sel.as<uint32_t>();
+    }
+    if ((rc = pinned(ctx->h_seeds, 4ull * (sd + 1) * std::max<uint64_t>(n_seeds_total, 1), "the seed list")))
+        return rc;
+    if (!noalloc && (rc = ctx->h_ctrl.reserve(sizeof(GhcmCtrl)))) return rc;
+    if ((rc = pinned(ctx->rec, sizeof(HostRec), "the progress record"))) return rc;
 
     uint32_t* hs = ctx->h_seeds.as<uint32_t>();
     {
@@ -465,10 +582,6 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 hs[k++] = i;
             }
     }
-    // jobs[]: the selected tiles plus the carried tiles a long selection hands to the tile
-    // launch, so one word per tile
-    if ((rc = ctx->sel.reserve(4ull * std::max<uint64_t>(1, T)))) return rc;
-    if (ctx->sel.bytes < 4ull * T) return fail(GHCM_ERR_INVALID_ARG, "selection buffer below %u words", T);
     // Grid of the device-mode solve kernel: at most one resident round of work-groups;
     // each work-group loops over list entries, so the grid never has to match the list.
     const uint32_t resident = (uint32_t)std::max(1, sk.blocks_per_cu()) * (uint32_t)std::max(1, ctx->cus);
@@ -481,27 +594,26 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     hr->totals_tag = 0u;      // this solve's tag is written by its own k_publish_totals
 
     SolveArgs a{};
-    a.inst = ctx->inst.as<GhcmInst>();
+    a.inst = d_inst;
     a.n_inst = n_inst;
-    a.tinst = n_inst > 1 ? ctx->tinst.as<uint32_t>() : nullptr;
-    a.qwave = ctx->qwave.as<uint32_t>();
-    a.key = ctx->key.as<uint32_t>();
-    a.mask = ctx->mask.as<uint32_t>();
-    a.tmin = ctx->tmin.as<float>();
-    a.list = ctx->list.as<uint32_t>();
+    a.tinst = d_tinst;
+    a.qwave = d_qwave;
+    a.key = d_key;
+    a.mask = d_mask;
+    a.tmin = d_tmin;
+    a.list = d_list;
     a.tiles = T;
     a.shard_cap = shard_cap;
-    a.tstat = ctx->tstat.as<uint32_t>();
-    a.jobs = ctx->sel.as<uint32_t>();
-    a.ctrl = ctx->ctrl.as<GhcmCtrl>();
+    a.tstat = d_tstat;
+    a.jobs = d_sel;
+    a.ctrl = d_ctrl;
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
 
     // The instance table and the seeds are read by the kernels from mapped pinned memory:
     // the whole solve issues no runtime copy.
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
-    HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel,
-                              ctx->totals.as<unsigned long long>(), st));
+    HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st));
     if (three_d)
         HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, st));
     else
@@ -513,9 +625,8 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     bool totals_ready = false;
     auto publish_totals = [&]() -> int {
         const uint32_t tag = ++ctx->solve_seq;
-        HIP_TRY(launch_reduce_stats(a.tstat, T, ctx->totals.as<unsigned long long>(), st));
-        HIP_TRY(launch_publish_totals(ctx->totals.as<unsigned long long>(), a.ctrl,
-                                      ctx->rec.dev_as<HostRec>(), tag, st));
+        HIP_TRY(launch_reduce_stats(a.tstat, T, d_totals, st));
+        HIP_TRY(launch_publish_totals(d_totals, a.ctrl, ctx->rec.dev_as<HostRec>(), tag, st));
         for (uint64_t spins = 0; hr->totals_tag != tag; ++spins) {
             if ((spins & 4095u) == 4095u) {
                 const hipError_t q = hipStreamQuery(st);
@@ -944,6 +1055,16 @@ extern "C" int ghcm_solve(ghcm_ctx* ctx, const void* field, int field_kind, uint
             std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
     return rc;
   });
+}
+
+extern "C" int ghcm_solve_device_noalloc(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,
+                                         uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
+                                         float* d_time, ghcm_stats* stats) {
+    if (!ctx) return fail(GHCM_ERR_INVALID_ARG, "ctx is null");
+    if (n_seeds > kNoAllocSeeds)
+        return fail(GHCM_ERR_INVALID_ARG, "allocation-free solve: %u seeds, at most %u", n_seeds, kNoAllocSeeds);
+    HostInst hi{d_field, d_time, w, h, seeds_xy, n_seeds};
+    return solve_core(ctx, field_kind, &hi, 1, stats, false, true);
 }
 
 extern "C" int ghcm_solve3d_device(ghcm_ctx* ctx, const void* d_field, int field_kind, uint32_t w,

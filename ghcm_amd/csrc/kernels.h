@@ -44,6 +44,14 @@ struct PersistentKernel {
 PersistentKernel get_persistent_kernel(int tile_w, int tile_h, int kind, int mode);
 
 hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s);
+
+// Scratch of the allocation-free solve: a static device array of kNoAllocScratchBytes per
+// GPU.  noalloc_scratch_init (device current) looks up its address on that device once;
+// noalloc_scratch returns it (null before the lookup).
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s);
 hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out,

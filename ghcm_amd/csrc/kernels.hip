@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "ghcm_dev.h"
@@ -2316,6 +2317,37 @@ hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds, uint32_t n_see
     if (n_seeds == 0) return hipSuccess;
     hipLaunchKernelGGL(k_seed, dim3((n_seeds + 63) / 64), dim3(64), 0, s, a, seeds, n_seeds, tw, th);
     return hipGetLastError();
+}
+
+// ---- scratch of the allocation-free solve (ghcm_solve_device_noalloc) ----
+// That entry point keeps all of its scheduler state (control block, instance record, tile
+// records, wave lists, selection, counters) in this file-scope array instead of buffers the
+// library allocates.  Each GPU holds its own copy at its own address, for the life of the
+// process; host code never passes the array's name to a kernel, only the address that
+// hipGetSymbolAddress returned for the device (noalloc_scratch_init, called with that device
+// current when a context is created there).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* noalloc_scratch(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 }  // namespace ghcm

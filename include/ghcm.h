@@ -171,6 +171,22 @@ int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind,
                       uint32_t w, uint32_t h, const uint32_t* seeds_xy, uint32_t n_seeds,
                       float* d_time, ghcm_stats* stats);
 
+/* ---- allocation-free solve ----------------------------------------------------
+ * No counterpart in the reference: ghcm_solve_device for callers that cannot allocate
+ * (tight serving loops, code that may not call hipMalloc, harnesses that forbid it).  Same
+ * arguments and results; it calls no hipMalloc, hipHostMalloc, hipFree or other allocating
+ * routine.  Its scheduler state lives in a static device array of 16 MiB per GPU, held
+ * until the process ends, whose address ghcm_ctx_create looks up once per device; the
+ * pinned records it uses are reserved by ghcm_ctx_create too.  Limits: the grid's tiles must
+ * fit the array (about 300,000 tiles: a 4096^2 grid in 8x8 tiles, larger grids in larger
+ * tiles; more is refused with GHCM_ERR_INVALID_ARG), at most 4,096 seeds, the device
+ * scheduler with launch_mode 0 or 4 and profile_kernels off (else GHCM_ERR_UNSUPPORTED).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU, from any
+ * context, race; each call starts by clearing what must start at zero. */
+int ghcm_solve_device_noalloc(ghcm_ctx* ctx, const void* d_field, int field_kind,
+                              uint32_t w, uint32_t h, const uint32_t* seeds_xy,
+                              uint32_t n_seeds, float* d_time, ghcm_stats* stats);
+
 /* ---- three-dimensional solve (the K = 3 generalisation of the update,
  * eikonal_solve<K>, src/eikonal.h:73-118 and data/eikonal_common.cl:221-285; the
  * reference declares K <= 3 but solves only 2-D grids) ----------------------------
