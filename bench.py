@@ -60,6 +60,30 @@ def read_traffic():
     return None
 
 
+def host_description() -> dict:
+    """The CPU the baselines run on: model name, logical CPUs (nproc) and the CPUs this
+    process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except OSError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "cpus_allowed": allowed}
+
+
+def _with_host(d: dict) -> dict:
+    d.update({k: v for k, v in host_description().items() if k in ("cpu_model", "nproc")})
+    return d
+
+
 def cpu_baseline(cost, seeds, trials: int):
     """The oracle's FP32 FMM (restating src/eikonal_fmm.cpp) on one host core."""
     import oracle
@@ -80,10 +104,10 @@ def cpu_baseline(cost, seeds, trials: int):
         except OSError:
             pass
     med = statistics.median(times)
-    return ref, {"value": round(W * H / med / 1e6, 3), "unit": "Mcells/s", "cores": 1,
-                 "kind": "port",
-                 "sample": f"full {W}x{H} workload, oracle FP32 FMM, {trials} trials, median "
-                           f"{med:.2f} s, pinned to one core"}
+    return ref, _with_host({"value": round(W * H / med / 1e6, 3), "unit": "Mcells/s", "cores": 1,
+                            "kind": "port",
+                            "sample": f"full {W}x{H} workload, oracle FP32 FMM, {trials} trials, "
+                                      f"median {med:.2f} s, pinned to one core"})
 
 
 BATCH_CANDIDATES = [  # autotuner search space for the batch (kept small so the bench stays short)
@@ -109,6 +133,28 @@ def host_cores() -> int:
         return 1
 
 
+def _oracle_map(fn, items):
+    """fn over items on the host's cores (ctypes releases the GIL)."""
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(max_workers=host_cores()) as ex:
+        return list(ex.map(fn, items))
+
+
+def accuracy_vs_fp64(results, refs64) -> dict:
+    """L_inf / max T and mean L1 of each result against its FP64 FMM, worst and mean over
+    the results (finite cells; a finite/INF mismatch counts)."""
+    import oracle
+    rel, l1, mism = [], [], 0
+    for t, r in zip(results, refs64):
+        m = oracle.error_metrics(t, r)
+        rel.append(m["linf"] / max(m["max_ref"], 1e-30))
+        l1.append(m["l1"])
+        mism += m["finiteness_mismatch"]
+    return {"linf_rel_vs_fp64": float(max(rel)), "linf_rel_vs_fp64_mean": float(sum(rel) / len(rel)),
+            "l1_vs_fp64_mean": float(sum(l1) / len(l1)), "finiteness_mismatch": mism,
+            "results": len(rel)}
+
+
 def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
     """512 uneven instances in one device wave loop vs the same solves one at a time."""
     import numpy as np
@@ -126,7 +172,7 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
         for a in args:
             ctx.solve_device(a[0], g.FIELD_COST_F32, a[2], a[3], a[4], a[1])
         t_plain = time.perf_counter() - t0
-        ref = [dt.download() for _, dt in dev]
+        ref = [dt.download() for _, dt in dev]          # the one-at-a-time results
         best, res = ctx.autotune(g.FIELD_COST_F32, args, BATCH_CANDIDATES, reps=1)
         log(f"bench: batch autotune -> {best}")
         prepared = ctx.prepare_batch(args)          # the instance array, built once
@@ -138,34 +184,42 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             ts.append(time.perf_counter() - t0)
         t_batch = statistics.median(ts)
         worst = 0.0
+        batched = []
         for (_, dt), r in zip(dev, ref):
             t = dt.download()
+            batched.append(t)
             fin = np.isfinite(r)
             if not np.array_equal(fin, np.isfinite(t)):
                 worst = float("inf")
-                break
-            if fin.any():
+            elif fin.any():
                 worst = max(worst, float(np.max(np.abs(t[fin] - r[fin])) / max(1e-30, r[fin].max())))
         for dc, dt in dev:
             dc.free()
             dt.free()
     # CPU: the oracle FMM over independent instances on every host core, on a sample.
-    import concurrent.futures as cf
-
     import oracle
     cores = host_cores()
     sample = insts[:64]
     t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
-        list(ex.map(lambda i: oracle.fmm_f32(i.cost, i.seeds), sample))
+    _oracle_map(lambda i: oracle.fmm_f32(i.cost, i.seeds), sample)
     t_cpu = time.perf_counter() - t0
+    # Accuracy of every batched result and every one-at-a-time result against the FP64 FMM
+    # (north_star: batched solves/s at equal accuracy against the FP64 oracle).
+    t0 = time.perf_counter()
+    refs64 = _oracle_map(lambda i: oracle.fmm_f64(i.cost.astype(np.float64), i.seeds), insts)
+    acc = {"batched": accuracy_vs_fp64(batched, refs64), "plain": accuracy_vs_fp64(ref, refs64),
+           "reference": f"oracle FP64 FMM of every instance ({time.perf_counter() - t0:.1f} s on "
+                        f"{cores} threads)"}
+    del refs64, batched, ref
     keys = ("tile_w", "tile_h", "kernel", "n_active", "inner_iter_cap", "launch_mode")
     return {"workload": f"{len(insts)} uneven instances, sides 64..2048, {cells / 1e6:.0f} Mcells, "
                         "ring or smooth, 1-8 sources (seed 4040)",
-            "cpu_baseline": {"value": round(len(sample) / t_cpu, 2), "unit": "solves/s",
-                             "cores": cores, "kind": "port",
-                             "sample": f"oracle FP32 FMM over the first {len(sample)} batch "
-                                       f"instances, {cores} threads, {t_cpu:.1f} s"},
+            "cpu_baseline": _with_host({"value": round(len(sample) / t_cpu, 2), "unit": "solves/s",
+                                        "cores": cores, "kind": "port",
+                                        "sample": f"oracle FP32 FMM over the first {len(sample)} "
+                                                  f"batch instances, {cores} threads, {t_cpu:.1f} s"}),
+            "accuracy": acc,
+            "linf_rel_vs_fp64": acc["batched"]["linf_rel_vs_fp64"],
             "solves_per_s": round(len(insts) / t_batch, 1),
             "plain_solves_per_s": round(len(insts) / t_plain, 1),
             "speedup_vs_plain": round(t_plain / t_batch, 2),
@@ -179,9 +233,9 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             "max_rel_diff_vs_plain": worst}
 
 
-def _serve_runs(srv, submit, n_requests: int, seed: int, load: float = 0.8) -> dict:
-    """Saturated run (every request submitted at once), then Poisson arrivals at `load`
-    times the saturated rate; latencies are submit -> result, measured by the library."""
+def _serve_runs(srv, submit, n_requests: int, seed: int, loads=(0.8,)) -> dict:
+    """Saturated run (every request submitted at once), then Poisson arrivals at each of
+    `loads` times the saturated rate; latencies are submit -> result, measured by the library."""
     import threading
 
     import numpy as np
@@ -191,49 +245,59 @@ def _serve_runs(srv, submit, n_requests: int, seed: int, load: float = 0.8) -> d
         results = [srv.wait(t) for t in tickets]
         t_sat = time.perf_counter() - t0
     sat_rate = n_requests / t_sat
-    rate = load * sat_rate
-    gaps = np.random.default_rng(seed).exponential(1.0 / rate, n_requests)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    tickets = [None] * n_requests
+    out = {"results": results, "saturated_solves_per_s": round(sat_rate, 1), "requests": n_requests,
+           "loads": {}}
+    for li, load in enumerate(loads):
+        rate = load * sat_rate
+        gaps = np.random.default_rng(seed + li).exponential(1.0 / rate, n_requests)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        tickets = [None] * n_requests
 
-    def client():
-        start = time.perf_counter()
+        def client():
+            start = time.perf_counter()
+            for i in range(n_requests):
+                d = arrivals[i] - (time.perf_counter() - start)
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = submit(i)
+
+        th = threading.Thread(target=client)
+        t0 = time.perf_counter()
+        th.start()
+        lat = []
         for i in range(n_requests):
-            d = arrivals[i] - (time.perf_counter() - start)
-            if d > 0:
-                time.sleep(d)
-            tickets[i] = submit(i)
-
-    th = threading.Thread(target=client)
-    t0 = time.perf_counter()
-    th.start()
-    lat = []
-    for i in range(n_requests):
-        while tickets[i] is None:
-            time.sleep(0.0002)
-        _, info = srv.wait(tickets[i])
-        lat.append(info["latency_us"] / 1e3)
-    th.join()
-    t_open = time.perf_counter() - t0
-    lat = np.array(lat)
-    return {"results": results,
-            "saturated_solves_per_s": round(sat_rate, 1),
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            _, info = srv.wait(tickets[i])
+            lat.append(info["latency_us"] / 1e3)
+        th.join()
+        t_open = time.perf_counter() - t0
+        lat = np.array(lat)
+        out["loads"][f"{load:.2f}"] = {
             "poisson_rate_per_s": round(rate, 1),
             "poisson_achieved_per_s": round(n_requests / t_open, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "max_ms": round(float(lat.max()), 3),
-            "peak_live": srv.stats()["peak_live"]}
+            "max_ms": round(float(lat.max()), 3)}
+    out["peak_live"] = srv.stats()["peak_live"]
+    return out
 
 
-def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 256,
-                  n_host_requests: int = 128) -> dict:
+def bench_serving(g, serve_cfg: dict, n_distinct: int = 128, n_requests: int = 2048,
+                  n_host_requests: int = 256) -> dict:
     """Continuous batching on requests already in HBM (device buffers) and on host buffers
-    (PCIe in and out), each against the same solves made one at a time."""
+    (PCIe in and out), each against the same solves made one at a time; every served result
+    of the distinct fields is checked against the FP64 FMM, and the oracle FMM on all host
+    cores is the CPU leg."""
     import numpy as np
+
+    import oracle
     from ghcm_amd import workloads
+    t0 = time.perf_counter()
     _, insts = workloads.serving_requests(n_distinct, 1.0)
+    log(f"bench: {n_distinct} serving fields generated in {time.perf_counter() - t0:.1f} s")
     pick = [i % n_distinct for i in range(n_requests)]
+    loads = (0.5, 0.8)
 
     def rel_diff(t, p):
         fin = np.isfinite(p)
@@ -241,10 +305,11 @@ def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 25
             return float("inf")
         return float(np.max(np.abs(t[fin] - p[fin])) / p[fin].max()) if fin.any() else 0.0
 
-    out = {"workload": f"{n_requests} requests ({n_host_requests} with host buffers) cycling "
-                       f"{n_distinct} square fields, sides log-uniform 128..4096, ring or smooth "
-                       "(sigma 32), 1-4 sources (seed 5050); Poisson arrivals at 80% of the "
-                       "saturated rate"}
+    cells = sum(r.w * r.h for r in insts)
+    out = {"workload": f"{n_requests} requests in HBM ({n_host_requests} more with host buffers) "
+                       f"cycling {n_distinct} square fields ({cells / 1e6:.0f} Mcells), sides "
+                       "log-uniform 128..4096, ring or smooth (sigma 32), 1-4 sources (seed "
+                       "5050); Poisson arrivals at 50% and 80% of the saturated rate"}
     with g.Context(0, **serve_cfg) as ctx:
         # ---- device-resident requests ----
         d_fields = [ctx.to_device(r.cost) for r in insts]
@@ -265,9 +330,12 @@ def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 25
                 return srv.submit_device(d_fields[k].ptr, r.w, r.h, r.seeds, d_times[i].ptr)
             for i in range(4):
                 srv.wait(submit_dev(i))
-            dev = _serve_runs(srv, submit_dev, n_requests, 5051)
+            dev = _serve_runs(srv, submit_dev, n_requests, 5051, loads)
         dev.pop("results")
-        worst = max(rel_diff(d_times[i].download(), plain[pick[i]]) for i in range(n_requests))
+        # the last request of every distinct field, after the last load
+        last = {pick[i]: i for i in range(n_requests)}
+        served = [d_times[last[k]].download() for k in range(n_distinct)]
+        worst = max(rel_diff(served[k], plain[k]) for k in range(n_distinct))
         dev.update({"plain_solves_per_s": round(1.0 / t_plain_dev, 1),
                     "speedup_vs_plain": round(dev["saturated_solves_per_s"] * t_plain_dev, 2),
                     "max_rel_diff_vs_plain": worst})
@@ -281,36 +349,101 @@ def bench_serving(g, serve_cfg: dict, n_distinct: int = 64, n_requests: int = 25
         for r in insts:
             ctx.solve(r.cost, r.seeds)
         t_plain_host = (time.perf_counter() - t0) / n_distinct
+    hpick = [i % n_distinct for i in range(n_host_requests)]
     with g.Server(0, **serve_cfg) as srv:
         costs = [r.cost for r in insts]
         for i in range(4):
             srv.wait(srv.submit(costs[i], insts[i].seeds))
-        host = _serve_runs(srv, lambda i: srv.submit(costs[pick[i]], insts[pick[i]].seeds),
-                           n_host_requests, 5052)
-    worst = max(rel_diff(t, plain[pick[i]]) for i, (t, _) in enumerate(host.pop("results")))
+        host = _serve_runs(srv, lambda i: srv.submit(costs[hpick[i]], insts[hpick[i]].seeds),
+                           n_host_requests, 5052, (0.8,))
+    worst = max(rel_diff(t, plain[hpick[i]]) for i, (t, _) in enumerate(host.pop("results")))
     host.update({"plain_solves_per_s": round(1.0 / t_plain_host, 1),
                  "speedup_vs_plain": round(host["saturated_solves_per_s"] * t_plain_host, 2),
                  "max_rel_diff_vs_plain": worst})
     out["host_buffers"] = host
+    # ---- accuracy against the FP64 FMM, and the CPU leg ----
+    refs64 = _oracle_map(lambda r: oracle.fmm_f64(r.cost.astype(np.float64), r.seeds), insts)
+    out["accuracy"] = {"served": accuracy_vs_fp64(served, refs64),
+                       "plain": accuracy_vs_fp64(plain, refs64),
+                       "reference": f"oracle FP64 FMM of each of the {n_distinct} distinct fields; "
+                                    "served = the last request of each field in HBM serving"}
+    out["linf_rel_vs_fp64"] = out["accuracy"]["served"]["linf_rel_vs_fp64"]
+    del refs64, served, plain
+    cores = host_cores()
+    t0 = time.perf_counter()
+    _oracle_map(lambda r: oracle.fmm_f32(r.cost, r.seeds), insts)
+    t_cpu = time.perf_counter() - t0
+    out["cpu_baseline"] = _with_host({"value": round(n_distinct / t_cpu, 2), "unit": "solves/s",
+                                      "cores": cores, "kind": "port",
+                                      "sample": f"oracle FP32 FMM over the {n_distinct} distinct "
+                                                f"fields, {cores} threads, {t_cpu:.1f} s"})
     return out
 
 
-def bench_precision(g, cfg: dict, speed, seeds, ref64) -> dict:
-    """The 4096^2 field stored as FP32 cost, FP32/FP16/BF16 speed (times FP32) vs the FP64 FMM."""
+def contrast_refs_async(g, pool):
+    """The 4096^2 contrast field (rings at speed 1e-4) and, on a worker thread pinned to the
+    last allowed core, its FP64 FMM and FP32 FIM (the FIM alone takes about 110 s on one
+    core): started early so it overlaps the GPU legs."""
+    from ghcm_amd import workloads
+    r = workloads.contrast_4096()
+
+    def work():
+        import oracle
+        try:
+            os.sched_setaffinity(0, {max(os.sched_getaffinity(0))})   # this thread only
+        except OSError:
+            pass
+        t0 = time.perf_counter()
+        c64 = oracle.fmm_f64(1.0 / r.speed, r.seeds)
+        cfim, _ = oracle.fim_f32(g.cost_from_speed(r.speed), r.seeds, 1e-5)
+        return c64, cfim, time.perf_counter() - t0
+
+    return r, pool.submit(work)
+
+
+def bench_precision(g, cfg: dict, speed, seeds, ref64, ref_fim, contrast=None) -> dict:
+    """The 4096^2 smooth field stored as FP32 cost and FP32/FP16/BF16 speed (times and updates
+    in FP32), and the 4096^2 contrast field (rings at speed 1e-4) as FP32/FP16/BF16 speed: L_inf
+    and L1 of each against the FP64 FMM and against the oracle's FP32 FIM (minimum change
+    1e-5), beside the FIM's own error against the FP64 FMM."""
     import numpy as np
 
     import oracle
-    out = {"reference": "oracle FP64 FMM", "tolerance_rel": 1e-4}
-    fields = [("cost_f32", g.cost_from_speed(speed), g.FIELD_COST_F32),
-              ("speed_f32", speed.astype(np.float32), g.FIELD_SPEED_F32),
-              ("speed_f16", speed.astype(np.float16), g.FIELD_SPEED_F16),
-              ("speed_bf16", g.to_bf16(speed.astype(np.float32)), g.FIELD_SPEED_BF16)]
+
+    def metrics(t, r64, rfim):
+        m = oracle.error_metrics(t, r64)
+        d = {"linf": m["linf"], "l1": m["l1"], "linf_rel": m["linf"] / m["max_ref"],
+             "finiteness_mismatch": m["finiteness_mismatch"]}
+        if rfim is not None:
+            mf = oracle.error_metrics(t, rfim.astype(np.float64))
+            d.update({"linf_vs_fim32": mf["linf"], "l1_vs_fim32": mf["l1"],
+                      "linf_rel_vs_fim32": mf["linf"] / m["max_ref"]})
+        return d
+
+    def stored(sp):
+        return [("speed_f32", sp.astype(np.float32), g.FIELD_SPEED_F32),
+                ("speed_f16", sp.astype(np.float16), g.FIELD_SPEED_F16),
+                ("speed_bf16", g.to_bf16(sp.astype(np.float32)), g.FIELD_SPEED_BF16)]
+
+    out = {"reference": "oracle FP64 FMM; oracle FP32 FIM (minimum change 1e-5)",
+           "tolerance_rel": 1e-4}
     with g.Context(0, **cfg) as ctx:
-        for name, field, kind in fields:
+        sm = {}
+        for name, field, kind in [("cost_f32", g.cost_from_speed(speed), g.FIELD_COST_F32)] + stored(speed):
             t, _ = ctx.solve(field, seeds, kind=kind)
-            m = oracle.error_metrics(t, ref64)
-            out[name] = {"linf": m["linf"], "l1": m["l1"], "linf_rel": m["linf"] / m["max_ref"],
-                         "finiteness_mismatch": m["finiteness_mismatch"]}
+            sm[name] = metrics(t, ref64, ref_fim)
+        sm["fim32_vs_fp64"] = metrics(ref_fim, ref64, None)
+        out["smooth_4096"] = sm
+        if contrast is not None:
+            r, fut = contrast
+            c64, cfim, t_ref = fut.result()
+            cf = {"workload": "4096^2, seven ring walls at speed 1e-4 on speed 1 (the 256^2 geometry "
+                              f"x16), source at the centre; FP64 FMM and FP32 FIM {t_ref:.0f} s on one core"}
+            for name, field, kind in stored(r.speed):
+                t, _ = ctx.solve(field, r.seeds, kind=kind)
+                cf[name] = metrics(t, c64, cfim)
+            cf["fim32_vs_fp64"] = metrics(cfim, c64, None)
+            out["contrast_4096"] = cf
     return out
 
 
@@ -507,6 +640,10 @@ def main() -> int:
     plain_cfg = dict(tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
                      inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,
                      launch_mode=args.launch_mode)
+    # The contrast field's oracle solutions run on one host core while the GPU legs below run.
+    import concurrent.futures as cf
+    bg = cf.ThreadPoolExecutor(max_workers=1)
+    contrast = None if args.no_precision else contrast_refs_async(g, bg)
     if not args.no_batch:
         log("bench: batched solves")
         out["batched"] = bench_batch(g, plain_cfg, 3)
@@ -521,9 +658,10 @@ def main() -> int:
             scfg.update(out["batched"]["autotuned"])
         out["serving"] = bench_serving(g, scfg)
         sv = out["serving"]
+        l80 = sv["device_resident"]["loads"]["0.80"]
         log(f"bench: serving (HBM-resident) {sv['device_resident']['saturated_solves_per_s']} "
             f"solves/s, {sv['device_resident']['speedup_vs_plain']}x plain, p50 "
-            f"{sv['device_resident']['p50_ms']} ms p99 {sv['device_resident']['p99_ms']} ms; "
+            f"{l80['p50_ms']} ms p99 {l80['p99_ms']} ms at 80% load; "
             f"host buffers {sv['host_buffers']['saturated_solves_per_s']} solves/s, "
             f"{sv['host_buffers']['speedup_vs_plain']}x plain")
 
@@ -541,10 +679,10 @@ def main() -> int:
         t0 = time.perf_counter()
         ref_fim, _ = oracle.fim_f32(cost, seeds, 1e-5)
         t_fim = time.perf_counter() - t0
-        out["cpu_baseline_fim"] = {"value": round(W * H / t_fim / 1e6, 3), "unit": "Mcells/s",
-                                   "cores": 1, "kind": "port",
-                                   "sample": f"full {W}x{H} workload, oracle FP32 FIM "
-                                             f"(minimum change 1e-5), 1 trial, {t_fim:.1f} s"}
+        out["cpu_baseline_fim"] = _with_host({"value": round(W * H / t_fim / 1e6, 3), "unit": "Mcells/s",
+                                              "cores": 1, "kind": "port",
+                                              "sample": f"full {W}x{H} workload, oracle FP32 FIM "
+                                                        f"(minimum change 1e-5), 1 trial, {t_fim:.1f} s"})
         err = float(np.max(np.abs(t_gpu.astype(np.float64) - ref)))
         fin = np.isfinite(ref_fim)
         err_fim = float(np.max(np.abs(t_gpu[fin].astype(np.float64) - ref_fim[fin])))
@@ -553,12 +691,17 @@ def main() -> int:
                          "tolerance_rel": 1e-4}
     else:
         out["cpu_baseline"] = None
+        ref_fim = None
     if not args.no_precision:
         import oracle
-        log("bench: storage precision vs the FP64 FMM")
+        log("bench: storage precision vs the FP64 FMM and the FP32 FIM")
         ref64 = oracle.fmm_f64(1.0 / speed, seeds)
-        out["precision"] = bench_precision(g, plain_cfg, speed, seeds, ref64)
+        if ref_fim is None:
+            ref_fim, _ = oracle.fim_f32(cost, seeds, 1e-5)
+        out["precision"] = bench_precision(g, plain_cfg, speed, seeds, ref64, ref_fim, contrast)
     ctx.close()
+    bg.shutdown(wait=True)
+    out["host"] = host_description()
     print(json.dumps(out), flush=True)
     return 0
 

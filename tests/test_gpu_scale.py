@@ -64,7 +64,9 @@ def test_plain4096_host_buffers(plain4096):
         assert np.max(np.abs(t.astype(np.float64) - t_dev)) <= 1e-5 * ref.max()
 
 
-@pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
+# About three times the errors measured on MI355X for this 512^2 field (sigma 8): FP32
+# 4.8e-7, FP16 2.8e-5, BF16 2.7e-4 of max T, with either kernel.
+@pytest.mark.parametrize("kind,tol", [("speed_f32", 1.5e-6), ("speed_f16", 8.5e-5), ("speed_bf16", 8e-4)])
 def test_precision_storage_vs_fp64(kind, tol):
     """Speed stored as FP32/FP16/BF16, times and updates in FP32, against the FP64 FMM."""
     speed = g.gen_smooth_speed(512, 512, 77, 8.0, 0.1, 10.0)
@@ -95,8 +97,9 @@ def test_ring_contrast_1e4_f16():
         with g.Context(0, tile_w=16, tile_h=16) as ctx:
             t, _ = ctx.solve(field, seeds, kind=g.KINDS[kind])
         m = oracle.error_metrics(t, ref64)
+        print(f"  contrast 512 {kind}: linf_rel={m['linf'] / m['max_ref']:.2e}", flush=True)
         assert m["finiteness_mismatch"] == 0
-        assert m["linf"] <= 1e-3 * m["max_ref"], (kind, m)
+        assert m["linf"] <= CONTRAST_TOL[kind] * m["max_ref"], (kind, m)
 
 
 def test_batch_equals_one_at_a_time():
@@ -189,13 +192,54 @@ def contrast_4096():
     return speed, seeds, oracle.fmm_f64(1.0 / speed, seeds)
 
 
-@pytest.mark.parametrize("kind,tol", [("speed_f32", 1e-4), ("speed_f16", 1e-3), ("speed_bf16", 8e-3)])
-def test_contrast_4096_vs_fp64(contrast_4096, kind, tol):
+# Storage-precision tolerances (L_inf / max T against the FP64 FMM), about three times the
+# errors measured on MI355X: 4096^2 smooth field FP32 1.0e-6, FP16 1.1e-5, BF16 1.1e-4; 4096^2
+# rings at speed 1e-4 FP32 4.1e-6, FP16 1.6e-4, BF16 1.3e-3.  FP32 also stays inside the
+# 1e-4 bar of the plain form.
+SMOOTH_TOL = {"speed_f32": 3e-6, "speed_f16": 5e-5, "speed_bf16": 4e-4}
+CONTRAST_TOL = {"speed_f32": 1.5e-5, "speed_f16": 5e-4, "speed_bf16": 4e-3}
+
+
+def _stored(speed, kind):
+    return {"speed_f32": lambda: speed.astype(np.float32),
+            "speed_f16": lambda: speed.astype(np.float16),
+            "speed_bf16": lambda: g.to_bf16(speed.astype(np.float32))}[kind]()
+
+
+@pytest.fixture(scope="module")
+def plain4096_refs(plain4096):
+    speed, cost, seeds = plain4096
+    return oracle.fmm_f64(1.0 / speed, seeds), oracle.fim_f32(cost, seeds, 1e-5)[0]
+
+
+@pytest.mark.parametrize("kind", ["speed_f32", "speed_f16", "speed_bf16"])
+def test_plain4096_storage_precision(plain4096, plain4096_refs, kind):
+    """The 4096^2 smooth field with its speed stored as FP32/FP16/BF16 (times and updates in
+    FP32): against the FP64 FMM within the storage tolerance, and against the oracle's FP32
+    FIM (the reference's second CPU solver) within that tolerance plus the FIM's own error."""
+    speed, cost, seeds = plain4096
+    ref64, ref_fim = plain4096_refs
+    print(f"case plain4096 storage {kind}", flush=True)
+    with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
+                   inner_iter_cap=1) as ctx:
+        t, st = ctx.solve(_stored(speed, kind), seeds, kind=g.KINDS[kind])
+    m = oracle.error_metrics(t, ref64)
+    m_fim = oracle.error_metrics(ref_fim, ref64)
+    d_fim = float(np.max(np.abs(t.astype(np.float64) - ref_fim)))
+    print(f"  {kind}: linf_rel={m['linf'] / m['max_ref']:.2e} l1={m['l1']:.3e} "
+          f"vs fim32 linf_rel={d_fim / m['max_ref']:.2e} (fim32 vs fp64 "
+          f"{m_fim['linf'] / m['max_ref']:.2e})", flush=True)
+    assert st["converged"] == 1 and m["finiteness_mismatch"] == 0
+    assert m["linf"] <= SMOOTH_TOL[kind] * m["max_ref"]
+    assert d_fim <= SMOOTH_TOL[kind] * m["max_ref"] + m_fim["linf"]
+
+
+@pytest.mark.parametrize("kind", ["speed_f32", "speed_f16", "speed_bf16"])
+def test_contrast_4096_vs_fp64(contrast_4096, kind):
     """1e-4..1 speed contrast at full size: every storage precision against the FP64 FMM."""
     speed, seeds, ref64 = contrast_4096
-    field = {"speed_f32": lambda: speed.astype(np.float32),
-             "speed_f16": lambda: speed.astype(np.float16),
-             "speed_bf16": lambda: g.to_bf16(speed.astype(np.float32))}[kind]()
+    field = _stored(speed, kind)
+    tol = CONTRAST_TOL[kind]
     print(f"case contrast 4096 {kind}", flush=True)
     with g.Context(0, tile_w=64, tile_h=64, kernel=g.KERNEL_SWEEP, n_active=512,
                    inner_iter_cap=1) as ctx:
