@@ -75,6 +75,7 @@ struct ghcm_ctx {
     std::vector<unsigned char> gkey;
     // device scratch
     DevBuf inst, tinst, qwave, key, mask, tmin, list, ctrl, seeds, jobs, status, tstat, totals, sel;
+    DevBuf edge, cedge;         // sweep kernel: edge columns of every tile (time, cost)
     DevBuf field_io, time_io;   // staging for the host-buffer entry point
     PinnedBuf io_stage[2];      // pinned parts of host-buffer copies (ping-pong)
     hipEvent_t ev_io[2] = {nullptr, nullptr};
@@ -332,7 +333,7 @@ extern "C" int ghcm_ctx_destroy(ghcm_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->inst, &ctx->tinst, &ctx->qwave, &ctx->key, &ctx->mask, &ctx->tmin,
                       &ctx->list, &ctx->ctrl, &ctx->seeds, &ctx->jobs, &ctx->status,
-                      &ctx->tstat, &ctx->totals, &ctx->sel,
+                      &ctx->tstat, &ctx->totals, &ctx->sel, &ctx->edge, &ctx->cedge,
                       &ctx->field_io, &ctx->time_io})
         b->release();
     for (PinnedBuf* b : {&ctx->h_ctrl, &ctx->h_seeds, &ctx->h_inst, &ctx->h_jobs,
@@ -402,9 +403,10 @@ int solve_core(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst, ghc
 // Byte offsets of the scheduler state of a T-tile solve inside the scratch array; each
 // region starts on a 256-byte boundary; `end` is the bytes used.
 struct ScratchLayout {
-    size_t ctrl, inst, totals, qwave, key, mask, tmin, tstat, sel, list, end;
+    size_t ctrl, inst, totals, qwave, key, mask, tmin, tstat, sel, list, edge, cedge, end;
 };
-ScratchLayout scratch_layout(uint32_t T, uint32_t shard_cap) {
+// edge_h: rows of a tile's edge columns (the sweep kernel's TH), 0 without edge columns.
+ScratchLayout scratch_layout(uint32_t T, uint32_t shard_cap, uint32_t edge_h = 0) {
     auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
     ScratchLayout l{};
     size_t o = 0;
@@ -418,16 +420,18 @@ ScratchLayout scratch_layout(uint32_t T, uint32_t shard_cap) {
     l.tstat = o;  o = up(o + 12ull * T);
     l.sel = o;    o = up(o + 4ull * std::max<uint32_t>(1, T));
     l.list = o;   o = up(o + 12ull * kShards * shard_cap);
+    l.edge = o;   o = up(o + 8ull * edge_h * T);
+    l.cedge = o;  o = up(o + 8ull * edge_h * T);
     l.end = o;
     return l;
 }
 
 // Largest tile count whose scheduler state fits the scratch array.
-uint32_t noalloc_max_tiles() {
+uint32_t noalloc_max_tiles(uint32_t edge_h) {
     uint32_t lo = 1, hi = 1u << 24;
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo + 1) / 2;
-        if (scratch_layout(mid, (mid + kShards - 1) / kShards).end <= kNoAllocScratchBytes) lo = mid;
+        if (scratch_layout(mid, (mid + kShards - 1) / kShards, edge_h).end <= kNoAllocScratchBytes) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -515,6 +519,9 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     HIP_TRY(hipSetDevice(ctx->ordinal));
     hipStream_t st = ctx->stream;
     const uint32_t shard_cap = (T + kShards - 1) / kShards;
+    // The sweep kernel stages its halo columns from edge columns kept for every tile.
+    const uint32_t edge_h = (!three_d && cfg.kernel == GHCM_KERNEL_SWEEP) ? th : 0u;
+    float *d_edge = nullptr, *d_cedge = nullptr;
     // Device scheduler state: the context's grow-only buffers, or the scratch array.
     GhcmInst* d_inst = nullptr;
     uint32_t *d_tinst = nullptr, *d_qwave = nullptr, *d_key = nullptr, *d_mask = nullptr;
@@ -525,11 +532,15 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     if (noalloc) {
         char* const base = static_cast<char*>(noalloc_scratch(ctx->ordinal));
         if (!base) return fail(GHCM_ERR_HIP, "allocation-free solve: no scratch address for device %d", ctx->ordinal);
-        const ScratchLayout l = scratch_layout(T, shard_cap);
+        const ScratchLayout l = scratch_layout(T, shard_cap, edge_h);
         if (l.end > kNoAllocScratchBytes)
             return fail(GHCM_ERR_INVALID_ARG, "allocation-free solve: %u tiles of %ux%u need %zu bytes of "
                                               "scratch, above its %zu (at most %u tiles)", T, tw, th, l.end,
-                        kNoAllocScratchBytes, noalloc_max_tiles());
+                        kNoAllocScratchBytes, noalloc_max_tiles(edge_h));
+        if (edge_h) {
+            d_edge = reinterpret_cast<float*>(base + l.edge);
+            d_cedge = reinterpret_cast<float*>(base + l.cedge);
+        }
         d_ctrl = reinterpret_cast<GhcmCtrl*>(base + l.ctrl);
         d_inst = reinterpret_cast<GhcmInst*>(base + l.inst);
         d_totals = reinterpret_cast<unsigned long long*>(base + l.totals);
@@ -567,6 +578,12 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
         d_totals = ctx->totals.as<unsigned long long>();
         d_ctrl = ctx->ctrl.as<GhcmCtrl>();
         d_sel = ctx->sel.as<uint32_t>();
+        if (edge_h) {
+            if ((rc = ctx->edge.reserve(8ull * edge_h * T))) return rc;
+            if ((rc = ctx->cedge.reserve(8ull * edge_h * T))) return rc;
+            d_edge = ctx->edge.as<float>();
+            d_cedge = ctx->cedge.as<float>();
+        }
     }
     if ((rc = pinned(ctx->h_seeds, 4ull * (sd + 1) * std::max<uint64_t>(n_seeds_total, 1), "the seed list")))
         return rc;
@@ -607,6 +624,9 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     a.tstat = d_tstat;
     a.jobs = d_sel;
     a.ctrl = d_ctrl;
+    a.edge = d_edge;
+    a.cedge = d_cedge;
+    a.edge_h = edge_h;
     a.eps = cfg.min_change;
     a.iter_cap = cfg.inner_iter_cap;
 
@@ -614,6 +634,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     // the whole solve issues no runtime copy.
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
     HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st));
+    HIP_TRY(launch_edges_init(a, 0, T, tw, th, kind, st));
     if (three_d)
         HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, st));
     else

@@ -140,6 +140,17 @@ struct SolveArgs {
     const HostJob*  host_jobs;
     HostStatus*     status;
     uint32_t        n_host_jobs;
+    // Edge columns of every tile (sweep kernel; null otherwise), TH floats per column:
+    //   edge[g][0][r]  time of tile g's first column (x = ox) at row oy + r,
+    //   edge[g][1][r]  time of its last column (x = ox + TW - 1), full-width tiles only;
+    //   cedge[g][s][r] the cost of the same cells, filled once per solve.
+    // A tile's left and right halo columns are its neighbours' edge columns: staging reads
+    // them as two contiguous runs instead of one cache line per row.  The time edges are
+    // written by the tile's own commit, seeds and initialisation, so they always equal the
+    // time field.
+    float*          edge;
+    const float*    cedge;
+    uint32_t        edge_h;  // TH of the edge columns (0: no edges)
 };
 
 }  // namespace ghcm

@@ -280,6 +280,112 @@ __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, i
     }
 }
 
+// Stage a sweep tile: the same PW x PH box as stage_tile (tile plus halo ring, out-of-grid
+// cells +INF), read so that no cache line is fetched for a few of its bytes.  The interior is
+// TH aligned rows of TW cells of the time and cost fields; the top and bottom halo rows are
+// read from the fields (the time rows with their corners); the left and right halo columns
+// come from the neighbours' edge columns (a.edge, a.cedge: TH contiguous floats each) instead
+// of one line per row.  Every global load is issued before the first LDS store.
+template <int TW, int TH, int KIND>
+__device__ __forceinline__ void stage_sweep_tile(const SolveArgs& a, const GhcmInst& in, uint32_t g,
+                                                 int tx, int ox, int oy, int tid, float* t_s,
+                                                 float* c_s) {
+    constexpr int NT = 256;
+    constexpr int PW = TW + 2, PH = TH + 2, P = TW * TH;
+    constexpr int NI = (P + NT - 1) / NT;        // interior cells per thread
+    constexpr int NHX = 2 * PW + 2 * TH;         // halo cells: two rows with corners, two columns
+    constexpr int NH = (NHX + NT - 1) / NT;
+    const int W = (int)in.w, H = (int)in.h;
+    // Addresses are a wave-uniform origin plus a 32-bit byte offset per cell (the width limit
+    // of the C-ABI keeps TH + 2 rows of a grid below 2^32 bytes).
+    const int64_t org = (int64_t)(oy - 1) * W + (ox - 1);
+    const char* const tb = reinterpret_cast<const char*>(in.time) + org * 4;
+    const char* const fb = static_cast<const char*>(in.field) + org * (int64_t)field_bytes<KIND>();
+    const bool has_l = tx > 0, has_r = ox + TW < W;
+    // left neighbour's last column and right neighbour's first column (unused at the borders)
+    const size_t lo = has_l ? ((size_t)g - 1) * 2 * TH + TH : 0, ro = has_r ? ((size_t)g + 1) * 2 * TH : 0;
+    const float* const el = a.edge + lo;
+    const float* const er = a.edge + ro;
+    const float* const cl = a.cedge + lo;
+    const float* const cr = a.cedge + ro;
+    float tv[NI], cv[NI], th[NH], ch[NH];
+    auto cell_off = [W](int bx, int by) { return (uint32_t)by * (uint32_t)W + (uint32_t)bx; };
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const int i = tid + NT * k;
+        const int r = i / TW, c = i % TW;
+        tv[k] = __builtin_inff();
+        cv[k] = __builtin_inff();
+        if (i < P && ox + c < W && oy + r < H) {
+            const uint32_t off = cell_off(c + 1, r + 1);
+            tv[k] = *gmem(reinterpret_cast<const float*>(tb + off * 4u));
+            cv[k] = load_cost_at<KIND>(fb + off * field_bytes<KIND>());
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int h = tid + NT * k;
+        th[k] = __builtin_inff();
+        ch[k] = __builtin_inff();
+        if (h < 2 * PW) {
+            // top (box row 0, y = oy - 1) or bottom (box row PH - 1, y = oy + TH) halo row
+            const int by = h < PW ? 0 : PH - 1, bx = h % PW;
+            const int gx = ox + bx - 1, gy = oy + by - 1;
+            if (gx >= 0 && gy >= 0 && gx < W && gy < H) {
+                const uint32_t off = cell_off(bx, by);
+                th[k] = *gmem(reinterpret_cast<const float*>(tb + off * 4u));
+                // the cost box's corners are never read (the border checks use its rows
+                // and columns only)
+                if (bx > 0 && bx < PW - 1) ch[k] = load_cost_at<KIND>(fb + off * field_bytes<KIND>());
+            }
+        } else if (h < NHX) {
+            const int j = h - 2 * PW, right = j >= TH ? 1 : 0, r = j - right * TH;
+            if (oy + r < H && (right ? has_r : has_l)) {
+                th[k] = *gmem(right ? er + r : el + r);
+                ch[k] = *gmem(right ? cr + r : cl + r);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const int i = tid + NT * k;
+        if (i < P) {
+            const int q = (i / TW + 1) * PW + (i % TW) + 1;
+            t_s[q] = tv[k];
+            c_s[q] = cv[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NH; ++k) {
+        const int h = tid + NT * k;
+        int q = -1;
+        if (h < 2 * PW) {
+            q = (h < PW ? 0 : PH - 1) * PW + h % PW;
+        } else if (h < NHX) {
+            const int j = h - 2 * PW, right = j >= TH ? 1 : 0, r = j - right * TH;
+            q = (r + 1) * PW + (right ? PW - 1 : 0);
+        }
+        if (q >= 0) {
+            t_s[q] = th[k];
+            c_s[q] = ch[k];
+        }
+    }
+}
+
+// Edge columns of tile g from its LDS box (after its last round): wave 0 writes the first
+// column, wave 1 the last one when the tile is full width (only then has it a right
+// neighbour that reads it).  Rows past the grid are never read and not written.
+template <int TW, int TH>
+__device__ __forceinline__ void write_edges(const SolveArgs& a, uint32_t g, int sx, int sy, int w,
+                                            int L, const float* t_s) {
+    constexpr int PW = TW + 2;
+    if (w == 0) {
+        for (int r = L; r < sy; r += 64) *gmem(a.edge + (size_t)g * 2 * TH + r) = t_s[(r + 1) * PW + 1];
+    } else if (w == 1 && sx == TW) {
+        for (int r = L; r < sy; r += 64) *gmem(a.edge + (size_t)g * 2 * TH + TH + r) = t_s[(r + 1) * PW + TW];
+    }
+}
+
 template <int TW, int TH, int KIND, int MODE, bool HOST>
 __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uint32_t wave,
                              uint32_t nxt, uint32_t p, uint32_t job_slot,
@@ -730,7 +836,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const int W = (int)in.w, H = (int)in.h;
     const int sx = tile_extent(W, ox, TW), sy = tile_extent(H, oy, TH);
 
-    stage_tile<PW, PH, kSweepThreads, KIND>(in, ox, oy, tid, t_s, c_s);
+    stage_sweep_tile<TW, TH, KIND>(a, in, g, tx, ox, oy, tid, t_s, c_s);
     __syncthreads();
     float told[CPT];
 #pragma unroll
@@ -803,6 +909,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             }
         }
     }
+    write_edges<TW, TH>(a, g, sx, sy, w, L, t_s);
     // Border check, wave w takes side w: would the neighbour's adjacent cell improve by
     // >= eps from this tile's edge (data/eikonal_ghcm_systolic.cl:8-41)?
     {
@@ -1462,6 +1569,39 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
     publish_slots<kSweepThreads>(a, s_prefix, cur ^ 1u, hist);
 }
 
+// A seed on a tile's first or last column also sets that tile's edge column.
+__device__ __forceinline__ void seed_edges(const SolveArgs& a, const GhcmInst& in, uint32_t x,
+                                           uint32_t y, uint32_t tw, uint32_t th) {
+    if (!a.edge) return;
+    const uint32_t t = tile_of_cell(in.tile_base, in.tiles_x, x, y, tw, th);
+    const uint32_t cx = x % tw, r = y % th;
+    if (cx == 0) gmem(a.edge)[(size_t)t * 2 * th + r] = 0.0f;
+    if (cx == tw - 1) gmem(a.edge)[(size_t)t * 2 * th + th + r] = 0.0f;
+}
+
+// Edge columns of tiles [t0, t0 + count): time +INF (the time fields were just set to +INF)
+// and the cost of the same cells (+INF past the grid).  One work-group of 2 * th threads per
+// tile, one cell each.
+template <int KIND>
+__global__ void k_edges_init(SolveArgs a, uint32_t t0, uint32_t count, uint32_t tw, uint32_t th) {
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const uint32_t g = t0 + k;
+        const uint32_t ii = a.n_inst == 1 ? 0u : a.tinst[g];
+        const GhcmInst in = a.inst[ii];
+        int tx, ty;
+        tile_xy(g - in.tile_base, in.tiles_x, tx, ty);
+        const uint32_t j = threadIdx.x, right = j >= th ? 1u : 0u, r = j - right * th;
+        const uint32_t x = (uint32_t)tx * tw + (right ? tw - 1 : 0u), y = (uint32_t)ty * th + r;
+        float c = __builtin_inff();
+        if (x < in.w && y < in.h)
+            c = load_cost_at<KIND>(static_cast<const char*>(in.field) +
+                                   ((size_t)y * in.w + x) * field_bytes<KIND>());
+        const size_t o = (size_t)g * 2 * th + j;
+        gmem(a.edge)[o] = __builtin_inff();
+        gmem(const_cast<float*>(a.cedge))[o] = c;
+    }
+}
+
 // Serving admission: seeds of newly admitted instances, (x, y, slot) triples.  Queues the
 // tiles of each seed and its 4 neighbours for the wave after the current one (they join
 // list[cur^1] like any activation) and clears `done` so an idle loop resumes.
@@ -1475,6 +1615,7 @@ __global__ void k_seed_serve(SolveArgs a, const uint32_t* seeds, uint32_t n_seed
     const uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], ii = seeds[3 * s + 2];
     const GhcmInst in = a.inst[ii];
     gmem(in.time)[(size_t)y * in.w + x] = 0.0f;
+    seed_edges(a, in, x, y, tw, th);
     const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
     for (int d = 0; d < 5; ++d) {
         const int px = (int)x + dx[d], py = (int)y + dy[d];
@@ -1653,6 +1794,7 @@ __global__ void k_seed(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds,
     const uint32_t x = seeds[3 * s], y = seeds[3 * s + 1], ii = seeds[3 * s + 2];
     const GhcmInst in = a.inst[ii];
     gmem(in.time)[(size_t)y * in.w + x] = 0.0f;
+    seed_edges(a, in, x, y, tw, th);
     const int dx[5] = {0, -1, 1, 0, 0}, dy[5] = {0, 0, 0, -1, 1};
     for (int d = 0; d < 5; ++d) {
         int px = (int)x + dx[d], py = (int)y + dy[d];
@@ -2194,6 +2336,20 @@ BfKernel get_bf_kernel(int tw, int th, int kind, int mode, bool wide) {
 
 hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s) {
     hipLaunchKernelGGL(k_wave_end, dim3(1), dim3(1024), 0, s, a, first);
+    return hipGetLastError();
+}
+
+hipError_t launch_edges_init(const SolveArgs& a, uint32_t t0, uint32_t count, uint32_t tw,
+                             uint32_t th, int kind, hipStream_t s) {
+    if (!a.edge || count == 0) return hipSuccess;
+    const unsigned blocks = count < 65536u ? count : 65536u;
+    const dim3 grid(blocks), block(2 * th);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_edges_init<0>, grid, block, 0, s, a, t0, count, tw, th); break;
+        case 1: hipLaunchKernelGGL(k_edges_init<1>, grid, block, 0, s, a, t0, count, tw, th); break;
+        case 2: hipLaunchKernelGGL(k_edges_init<2>, grid, block, 0, s, a, t0, count, tw, th); break;
+        default: hipLaunchKernelGGL(k_edges_init<3>, grid, block, 0, s, a, t0, count, tw, th); break;
+    }
     return hipGetLastError();
 }
 

@@ -208,6 +208,7 @@ struct ghcm_server {
     hipEvent_t ev_upload = nullptr;
     hipGraphExec_t gexec = nullptr;
     DevBuf arena, inst, tinst, qwave, key, mask, tmin, list, tstat, sel, ctrl, sgen, totals;
+    DevBuf edge, cedge;                  // sweep kernel: edge columns of every tile of the pool
     static constexpr int kSeedRing = 8;   // seed staging buffers, reused round robin
     PinnedBuf h_seeds[kSeedRing], rec;
     hipEvent_t ev_seed[kSeedRing] = {};  // recorded after the launch that read each buffer
@@ -289,6 +290,10 @@ int ghcm_server::init() {
     if ((rc = sgen.reserve(4ull * slots))) return rc;
     if ((rc = totals.reserve(3 * sizeof(unsigned long long)))) return rc;
     if ((rc = rec.reserve(sizeof(HostRec)))) return rc;
+    if (cfg.kernel == GHCM_KERNEL_SWEEP) {
+        if ((rc = edge.reserve(8ull * th * T))) return rc;
+        if ((rc = cedge.reserve(8ull * th * T))) return rc;
+    }
     for (int k = 0; k < kSeedRing; ++k) {
         if ((rc = h_seeds[k].reserve(12 * 1024))) return rc;
         HIP_TRY(hipEventCreateWithFlags(&ev_seed[k], hipEventDisableTiming));
@@ -312,6 +317,11 @@ int ghcm_server::init() {
     a.rec = rec.dev_as<HostRec>();
     a.sgen = sgen.as<uint32_t>();
     a.n_slots = slots;
+    if (cfg.kernel == GHCM_KERNEL_SWEEP) {
+        a.edge = edge.as<float>();
+        a.cedge = cedge.as<float>();
+        a.edge_h = th;
+    }
 
     // Empty loop: lists empty, every slot generation 0 (no request carries it).
     std::memset(rec.p, 0, sizeof(HostRec));
@@ -360,7 +370,7 @@ void ghcm_server::release_all() {
     for (hipStream_t s : {stream, up_stream, down_stream})
         if (s) (void)hipStreamSynchronize(s);
     for (DevBuf* b : {&arena, &inst, &tinst, &qwave, &key, &mask, &tmin, &list, &tstat, &sel, &ctrl,
-                      &sgen, &totals})
+                      &sgen, &totals, &edge, &cedge})
         b->release();
     for (int k = 0; k < kSeedRing; ++k) {
         h_seeds[k].release();
@@ -476,6 +486,8 @@ int ghcm_server::admit(std::vector<Request*>& batch, uint32_t enqueued) {
         if (next_gen == 0) next_gen = 1;
         HIP_TRY(launch_admit_tiles(a, g, r->slot, r->gen, stream));
         HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(g.time), cells, kInfBits, stream));
+        // edge columns of the slot's tiles (sweep kernel), after its entry and tile map
+        HIP_TRY(launch_edges_init(a, g.tile_base, g.tiles_x * g.tiles_y, tw, th, kind, stream));
         for (size_t s = 0; s + 1 < r->seeds.size(); s += 2) {
             hs[k++] = r->seeds[s];
             hs[k++] = r->seeds[s + 1];
