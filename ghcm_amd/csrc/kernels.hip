@@ -620,17 +620,25 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             stat_add(&a.tstat[2 * (size_t)a.tiles + g], improved);
         }
     } else {
-        if (lane == 0) {
+        // Lanes 0-3 queue the dirty sides' neighbours and lane 4 the tile itself when the pass
+        // cap stopped it early, in one call, so their atomics' round trips overlap.
+        bool act = false;
+        uint32_t nb = g, bits = kMaskOpenAll;
+        float key = tmin;
+        if (lane < 4 && ((side_flags >> lane) & 1u)) {
+            act = true;
+            nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, lane);
+            bits = border_bit_across(lane);
+            key = side_key[lane & 3];
+        } else if (lane == 4) {
+            act = unfinished;
+        }
+        if (act) activate(a, nb, wave, nxt, p, bits, key);
+        if (lane == 5) {
             a.tmin[g] = tmin;
             stat_add(&a.tstat[g], 1u);
             stat_add(&a.tstat[(size_t)a.tiles + g], passes);
             stat_add(&a.tstat[2 * (size_t)a.tiles + g], improved);
-            if (unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tmin);
-        }
-        if (lane < 4 && ((side_flags >> lane) & 1u)) {
-            const int s = lane;
-            const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, s);
-            activate(a, nb, wave, nxt, p, border_bit_across(s), side_key[s]);
         }
     }
 }
@@ -947,17 +955,12 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             s_f32[side] = kc;
         }
     }
-    if constexpr (!HOST) {
-        // Activate the dirty sides' neighbours now, so the atomics' round trips overlap the
-        // commit below; the neighbours run in a later wave, after this kernel's stores.
-        __syncthreads();
-        if (tid < 4 && s_u32[4 + tid]) {
-            const int sd = tid;
-            const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, sd);
-            activate(a, nb, wave, nxt, p, border_bit_across(sd), s_f32[sd]);
-        }
-    }
-
+    // Per-wave tile minimum and improved count, then one barrier after which every
+    // activation is issued at once: the dirty sides' neighbours (lanes 0-3 of wave 0) and,
+    // when the round cap stopped the tile before it converged, the tile itself (wave 1).  Each
+    // activation is two dependent atomics; issued side by side their round trips overlap
+    // instead of following one another at the end of the visit.  The neighbours run in a
+    // later wave, after this kernel's stores.
     tmin = wave_min(tmin);
     improved = wave_sum(improved);
     if (L == 0) {
@@ -965,8 +968,16 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         s_u32[8 + w] = improved;
     }
     __syncthreads();
-    if (tid == 0) {
-        const float tm = fminf(fminf(s_f32[4], s_f32[5]), fminf(s_f32[6], s_f32[7]));
+    const float tm = fminf(fminf(s_f32[4], s_f32[5]), fminf(s_f32[6], s_f32[7]));
+    if constexpr (!HOST) {
+        if (tid < 4 && s_u32[4 + tid]) {
+            const int sd = tid;
+            const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, sd);
+            activate(a, nb, wave, nxt, p, border_bit_across(sd), s_f32[sd]);
+        }
+        if (tid == 64 && unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
+    }
+    if (tid == 128) {
         const uint32_t imp = s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11];
         a.tmin[g] = tm;
         stat_add(&a.tstat[g], 1u);
@@ -984,8 +995,6 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
             st.smallest_time = sm;
             st.requeue = unfinished ? 1u : 0u;
             a.status[job_slot] = st;
-        } else {
-            if (unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
         }
     }
 }
