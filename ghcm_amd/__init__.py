@@ -57,7 +57,7 @@ class Stats(ctypes.Structure):
                 ("inner_iterations", ctypes.c_uint64), ("cells_improved", ctypes.c_uint64),
                 ("converged", ctypes.c_uint32), ("tile_w", ctypes.c_uint32),
                 ("tile_h", ctypes.c_uint32), ("n_active", ctypes.c_uint32),
-                ("bytes_per_visit", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("bytes_per_visit", ctypes.c_uint32), ("tile_d", ctypes.c_uint32),
                 ("solve_kernel_us", ctypes.c_double), ("solve_launches", ctypes.c_uint64)]
 
     def to_dict(self) -> dict:

@@ -49,11 +49,12 @@ inline uint32_t bytes_per_visit(uint32_t tw, uint32_t th, int kind) {
     return P * (uint32_t)kind_bytes(kind) + (P + 2 * (tw + th)) * 4 + P * 4 + 40;
 }
 
-// The same for an 8x8x8 tile of a 3-D solve: field + time (tile and its 6 halo faces) +
-// time write + 40 B.
-inline uint32_t bytes_per_visit3d(uint32_t t, int kind) {
-    const uint32_t P = t * t * t;
-    return P * (uint32_t)kind_bytes(kind) + (P + 6 * t * t) * 4 + P * 4 + 40;
+// The same for a tw x th x td tile of a 3-D solve: field + time (tile and its 6 halo
+// faces) + time write + 40 B.
+inline uint32_t bytes_per_visit3d(uint32_t tw, uint32_t th, uint32_t td, int kind) {
+    const uint32_t P = tw * th * td;
+    const uint32_t faces = 2 * (tw * th + th * td + tw * td);
+    return P * (uint32_t)kind_bytes(kind) + (P + faces) * 4 + P * 4 + 40;
 }
 
 #define HIP_TRY(expr)                                                                        \

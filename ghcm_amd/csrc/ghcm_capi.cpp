@@ -459,8 +459,10 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                         want, b.bytes);
         return GHCM_OK;
     };
-    const uint32_t tw = three_d ? kTile3D : cfg.tile_w, th = three_d ? kTile3D : cfg.tile_h;
-    const uint32_t td = three_d ? kTile3D : 1u;
+    // 3-D solves: the sweep kernel's 64 x 8 x 8 tiles, or 8 x 8 x 8 tiles of in-tile passes.
+    const bool sweep3 = three_d && cfg.kernel == GHCM_KERNEL_SWEEP;
+    uint32_t tw = cfg.tile_w, th = cfg.tile_h, td = 1u;
+    if (three_d) tile3d_shape(sweep3, tw, th, td);
     const bool host_sched = cfg.scheduler == GHCM_SCHED_HOST_PQ;
     if (three_d && host_sched)
         return fail(GHCM_ERR_UNSUPPORTED, "3-D solves run with the device scheduler only");
@@ -469,7 +471,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     // Waves far wider than the chip (batches of many instances) take the higher-occupancy
     // form of the tile kernel where one is built.
     const bool wide = (uint64_t)n_sel >= 4ull * (uint64_t)std::max(1, ctx->cus);
-    const SolveKernel sk = three_d ? get_solve3d_kernel(kind)
+    const SolveKernel sk = three_d ? get_solve3d_kernel(kind, sweep3)
                                    : get_solve_kernel((int)tw, (int)th, kind, cfg.kernel, host_sched, wide);
     const solve_launcher_t launch = sk.launch;
     if (!launch)
@@ -636,7 +638,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st));
     HIP_TRY(launch_edges_init(a, 0, T, tw, th, kind, st));
     if (three_d)
-        HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, st));
+        HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, td, st));
     else
         HIP_TRY(launch_seed(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, st));
 
@@ -981,8 +983,9 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
         stats->converged = converged ? 1u : 0u;
         stats->tile_w = tw;
         stats->tile_h = th;
+        stats->tile_d = td;
         stats->n_active = n_sel;
-        stats->bytes_per_visit = three_d ? bytes_per_visit3d(kTile3D, kind) : bytes_per_visit(tw, th, kind);
+        stats->bytes_per_visit = three_d ? bytes_per_visit3d(tw, th, td, kind) : bytes_per_visit(tw, th, kind);
         stats->solve_kernel_us = kern_us;
         stats->solve_launches = kern_launches;
     }

@@ -21,9 +21,11 @@ struct SolveKernel {
 // waves hold many more tiles than CUs (selects a higher-occupancy form where one is built).
 SolveKernel get_solve_kernel(int tile_w, int tile_h, int kind, int mode, bool host,
                              bool wide = false);
-// Three-dimensional tile solve (8 x 8 x 8 tiles, device scheduler) for a field kind.
-SolveKernel get_solve3d_kernel(int kind);
-constexpr uint32_t kTile3D = 8;
+// Three-dimensional tile solve (device scheduler) for a field kind: sweep, 64 x 8 x 8 tiles
+// relaxed by the eight octant sweeps; otherwise 8 x 8 x 8 tiles relaxed by in-tile passes.
+SolveKernel get_solve3d_kernel(int kind, bool sweep);
+// Tile extents (x, y, z) of the 3-D kernels.
+void tile3d_shape(bool sweep, uint32_t& tw, uint32_t& th, uint32_t& td);
 // Tile solve with the wave boundary folded in (one launch per wave); sweep kernel only.
 SolveKernel get_fused_kernel(int tile_w, int tile_h, int kind, int mode);
 
@@ -78,6 +80,6 @@ hipError_t launch_seed(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n
                        uint32_t tile_w, uint32_t tile_h, hipStream_t s);
 // Seeds of a 3-D solve: (x, y, z, instance) quadruples in device-readable memory.
 hipError_t launch_seed3d(const SolveArgs& a, const uint32_t* seeds_xyzi, uint32_t n_seeds,
-                         hipStream_t s);
+                         uint32_t tile_w, uint32_t tile_h, uint32_t tile_d, hipStream_t s);
 
 }  // namespace ghcm

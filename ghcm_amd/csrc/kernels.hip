@@ -1875,6 +1875,32 @@ __device__ __forceinline__ float godunov3_raw(float a, float b, float c, float f
     return m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
 }
 
+// godunov3_raw with both candidates formed and the result selected, for the sweep step, where
+// the lanes of a wave take different branches and every branch would run anyway.  The same
+// operations in the same order: the result equals godunov3_raw's bit for bit.  The inputs are
+// non-negative or +INF, so the sort uses integer minimum and maximum.
+__device__ __forceinline__ float godunov3_sel(float a, float b, float c, float f) {
+    // Sorted by three-operand minimum, median and maximum of the bit patterns.
+    const uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b), uc = __float_as_uint(c);
+    const float m1 = __uint_as_float(min(min(ua, ub), uc));
+    const float m3 = __uint_as_float(max(max(ua, ub), uc));
+    const float m2 = __uint_as_float(max(min(ua, ub), min(max(ua, ub), uc)));
+    const float t1 = m1 + f;
+    // godunov_raw(m1, m2, m1, f) with its |m1 - m2| formed as m2 - m1 (the same value: the
+    // inputs are sorted), shared with the three-axis form.
+    const float u = m2 - m1;
+    const float s2 = 1.41421356237309515f * f;
+    const float two = __builtin_fmaf(__builtin_amdgcn_sqrtf((s2 - u) * (s2 + u)), 0.5f, 0.5f * u);
+    const float t2 = m1 + ((f >= u) ? two : f);
+    const float v = m3 - m1, w = u - v;
+    const float disc = fmaxf(3.0f * f * f - (u * u + v * v + w * w), 0.0f);
+    const float t3 = m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
+    // Both candidates exist here on every lane, so the selection is two v_cndmask, not a
+    // branch around their computation (a branch adds scalar instructions to every step).
+    __asm__ volatile("" ::"v"(t2), "v"(t3));
+    return !(t1 > m2) ? t1 : !(t2 > m3) ? t2 : t3;
+}
+
 constexpr int kT3 = 8;                 // 3-D tile edge
 constexpr int kT3Threads = 256;        // two cells per thread
 constexpr int kT3Box = kT3 + 2;        // staged box edge (tile plus a one-cell halo)
@@ -2096,9 +2122,421 @@ __global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8)))
     if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
 }
 
+// ---------------------------------------------------------------------------------------
+// Three-dimensional sweep tile solver: eight wave64s per work-group relax one 64 x 8 x 8 tile
+// with the eight Gauss-Seidel orders of 3-D fast sweeping (Zhao 2005), one octant per wave,
+// concurrently on the same LDS box.  Lane L owns the row (y, z) = (L mod 8, L / 8) and walks
+// along x (towards +x or -x with the octant); the octant's rank of the row, r = y' + z' (y'
+// and z' counted in the octant's directions, 0..14), skews the walk: at step s the lane is at
+// x-position (s - r) mod 64.  The rows upwind in y and z have rank r - 1, so they relaxed the
+// same x one step earlier: their results arrive through a DPP row shift (y, the neighbouring
+// lane) and ds_bpermute (z, eight lanes away), the upwind x value is the lane's own previous
+// result, and each is merged with an LDS read of that cell (improvements of the other seven
+// sweeps).  64 + 14 steps give every row one complete ordered walk, so a round of the eight
+// sweeps is one full fast-sweeping iteration over the tile in 78 steps; before and after its
+// walk a lane relaxes cells of the neighbouring walks instead of idling (every value involved
+// is an upper bound).  Rounds repeat until no cell improves by >= eps or the round cap.
+// ---------------------------------------------------------------------------------------
+constexpr int kS3X = 64, kS3Y = 8, kS3Z = 8;       // tile extents
+constexpr int kS3Threads = 512;                    // eight wave64s, one octant each
+constexpr int kS3PX = kS3X + 2;                    // box row: x = -1 .. 64
+// Box plane stride in words (>= 10 rows of 66).  A step's 64 lanes address the cells
+// (z, y, x) = (z, y, c -+ y -+ z); with rows of 66 words and planes of 695 the 64 addresses
+// fall in 64 different banks for half of the octants and at most two per bank for the others
+// (no stride from 660 to 800 does better for all eight).
+constexpr int kS3Plane = 695;
+constexpr int kS3Box = (kS3Z + 2) * kS3Plane;      // words of the time box; the cost box follows
+constexpr int kS3MaxRank = (kS3Y - 1) + (kS3Z - 1);
+constexpr uint32_t kS3HardRounds = 64;
+static_assert(kS3Y == 8 && kS3Z == 8, "lane L owns row (L mod 8, L / 8)");
+
+__device__ __forceinline__ int s3_box(int bx, int by, int bz) { return bz * kS3Plane + by * kS3PX + bx; }
+
+// One round of octant (DX, SY, SZ) over the LDS box (time box at shb, cost box kS3Box words
+// after it), run by one wave64.
+template <int DX, int SY, int SZ>
+__device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
+    constexpr int PX = kS3PX, PL = kS3Plane;
+    // Word offsets from the corner (x-1, y-1, z-1) of the cell's 3 x 3 x 3 neighbourhood: every
+    // read and the merge are immediate offsets from one per-step address.
+    constexpr int kCell = PL + PX + 1;
+    constexpr int kUpX = kCell - DX;
+    constexpr int kUpY = kCell - SY * PX;
+    constexpr int kUpZ = kCell - SZ * PL;
+    constexpr int kCost = kS3Box + kCell;
+    // Rows whose upwind y (z) neighbour is in the halo: y = 0 (7) for SY > 0 (< 0), z = 0 (7).
+    constexpr uint64_t kYHalo = SY > 0 ? 0x0101010101010101ull : 0x8080808080808080ull;
+    constexpr uint64_t kZHalo = SZ > 0 ? 0x00000000000000FFull : 0xFF00000000000000ull;
+    // DPP row shift by one lane towards higher lanes (SY > 0: lane L reads lane L-1) or lower.
+    constexpr int kDppY = SY > 0 ? 0x111 : 0x101;
+    const int y = L & 7, z = L >> 3;
+    const int rank = (SY > 0 ? y : kS3Y - 1 - y) + (SZ > 0 ? z : kS3Z - 1 - z);
+    const int zsrc = 4 * ((L - 8 * SZ) & 63);           // ds_bpermute byte address of the z-upwind lane
+    const int x0 = DX > 0 ? 0 : kS3X - 1;
+    const int rs = 4 * (z * PL + y * PX + x0);           // the walk's first cell (corner-relative)
+    // The first load steps to x-position (0 - rank) mod 64, or to rs where that is 0.
+    int bb = rs + 4 * DX * (kS3X - 1 - rank);
+    // 78 steps complete every walk; the loop runs 80 (four per iteration, never unrolled further).
+    constexpr int S = kS3X + kS3MaxRank;
+    constexpr int SE = (S + 3) & ~3;
+    struct Nb {
+        float ux, uy, uz, cc;
+        int b;
+        bool first;
+    };
+    // Single-word loads at immediate offsets from the step's address (relaxed atomic loads are
+    // not paired into ds_read2, which would need a second address register per step).
+    auto at = [&](int b, int word) -> float {
+        return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(shb + b + 4 * word),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+    };
+    // Step s: lanes with rank == s mod 64 are at x-position 0 (their walk restarts at rs).
+    auto load = [&](Nb& o, int st) {
+        o.first = rank == (st & 63);
+        bb = o.first ? rs : bb + 4 * DX;
+        o.b = bb;
+        o.ux = at(bb, kUpX);
+        o.uy = at(bb, kUpY);
+        o.uz = at(bb, kUpZ);
+        o.cc = at(bb, kCost);
+    };
+    auto umin = [](float u, float v) {
+        return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
+    };
+    const float inf = __builtin_inff();
+    float prev = inf;
+    auto step = [&](const Nb& n) {
+        const float yv = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(inf), __float_as_int(prev),
+                                                                    kDppY, 0xF, 0xF, false));
+        const float zv = __int_as_float(__builtin_amdgcn_ds_bpermute(zsrc, __float_as_int(prev)));
+        const float ax = umin(n.ux, n.first ? inf : prev);
+        const float ay = umin(n.uy, __builtin_amdgcn_inverse_ballot_w64(kYHalo) ? inf : yv);
+        const float az = umin(n.uz, __builtin_amdgcn_inverse_ballot_w64(kZHalo) ? inf : zv);
+        const float t = godunov3_sel(ax, ay, az, n.cc);
+        __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * kCell), __float_as_uint(t),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        prev = t;
+    };
+    // Loads of step s+1 are issued before step s is computed (two register sets): their cells
+    // are upper bounds whatever the interleaving, and the values step s produces reach step
+    // s+1 through registers.
+    Nb A, B;
+    load(A, 0);
+#pragma clang loop unroll_count(2)
+    for (int st = 0; st < SE; st += 2) {
+        load(B, st + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        step(A);
+        load(A, st + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        step(B);
+    }
+}
+
+// Stage the (kS3X + 2) x 10 x 10 box of 3-D sweep tile (ox, oy, oz): the time box without its
+// eight corners, the cost box at the tile and its six faces (edges and corners are never read).
+// Rows of the box along x are read as aligned runs of 64 cells (the tile's x range), one run
+// per wave-uniform row job; the x = -1 and x = 64 words of the rows are read one per thread.
+// Out-of-grid cells read +INF.  Every global load is issued before the first LDS store.
+template <int KIND>
+__device__ __forceinline__ void stage_sweep3(const GhcmInst& in, int ox, int oy, int oz, int tid,
+                                             float* t_s, float* c_s) {
+    constexpr int kRowJobs = 64 + 64 + 36 + 32;        // time, cost interior; time, cost halo rows
+    constexpr int NR = (kRowJobs + 7) / 8;
+    constexpr int kHaloWords = 2 * 100 + 2 * 64;       // time: every box row; cost: interior rows
+    // The wave index stays a vector value here: as a scalar, the 25 row jobs' addresses
+    // exceed the scalar registers and spill.
+    const int w = tid >> 6, L = tid & 63;
+    const int W = (int)in.w, H = (int)in.h, D = (int)in.d;
+    const char* const tb = reinterpret_cast<const char*>(in.time);
+    const char* const fb = static_cast<const char*>(in.field);
+    constexpr int fbytes = field_bytes<KIND>();
+    // Row job q -> field and box row (by, bz); false past the last job.  Jobs 0-63: time
+    // rows of the tile; 64-127: its cost rows; 128-163: time halo rows (the y-halo rows of the
+    // eight tile planes, then both z-halo planes whole); 164-195: cost halo rows (the same
+    // face rows without the box edges).
+    auto row_job = [](int q, bool& cost, int& by, int& bz) -> bool {
+        if (q < 128) {
+            cost = q >= 64;
+            const int r = q & 63;
+            by = (r & 7) + 1;
+            bz = (r >> 3) + 1;
+            return true;
+        }
+        q -= 128;
+        if (q < 36) {
+            cost = false;
+            if (q < 16) { by = (q & 1) ? kS3Y + 1 : 0; bz = (q >> 1) + 1; }
+            else        { q -= 16; bz = q >= 10 ? kS3Z + 1 : 0; by = q % 10; }
+            return true;
+        }
+        q -= 36;
+        if (q < 32) {
+            cost = true;
+            if (q < 16) { by = (q & 1) ? kS3Y + 1 : 0; bz = (q >> 1) + 1; }
+            else        { q -= 16; bz = q >= 8 ? kS3Z + 1 : 0; by = (q & 7) + 1; }
+            return true;
+        }
+        return false;
+    };
+    float v[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        bool cost = false;
+        int by = 0, bz = 0;
+        v[m] = __builtin_inff();
+        if (row_job(w + 8 * m, cost, by, bz)) {
+            const int gx = ox + L, gy = oy + by - 1, gz = oz + bz - 1;
+            if (gx < W && gy >= 0 && gz >= 0 && gy < H && gz < D) {
+                const size_t idx = ((size_t)gz * (size_t)H + (size_t)gy) * (size_t)W + (size_t)gx;
+                v[m] = cost ? load_cost_at<KIND>(fb + idx * fbytes)
+                            : *gmem(reinterpret_cast<const float*>(tb + idx * 4));
+            }
+        }
+    }
+    float hv = __builtin_inff();
+    int hq = -1;
+    bool hcost = false;
+    if (tid < kHaloWords) {
+        int by, bz, bx;
+        if (tid < 200) {
+            const int q = tid >> 1;
+            by = q % 10;
+            bz = q / 10;
+        } else {
+            const int q = (tid - 200) >> 1;
+            by = (q & 7) + 1;
+            bz = (q >> 3) + 1;
+            hcost = true;
+        }
+        bx = (tid & 1) ? kS3X + 1 : 0;
+        hq = s3_box(bx, by, bz);
+        const int gx = ox + bx - 1, gy = oy + by - 1, gz = oz + bz - 1;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < W && gy < H && gz < D) {
+            const size_t idx = ((size_t)gz * (size_t)H + (size_t)gy) * (size_t)W + (size_t)gx;
+            hv = hcost ? load_cost_at<KIND>(fb + idx * fbytes) : *gmem(reinterpret_cast<const float*>(tb + idx * 4));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        bool cost = false;
+        int by = 0, bz = 0;
+        if (row_job(w + 8 * m, cost, by, bz)) (cost ? c_s : t_s)[s3_box(L + 1, by, bz)] = v[m];
+    }
+    if (hq >= 0) (hcost ? c_s : t_s)[hq] = hv;
+}
+
+// One visit of 3-D sweep tile g: stage, rounds of the eight octant sweeps, commit, face
+// predictions for the six neighbours, activations.  Thread (wave w, lane L) owns the cells
+// (x = L, y = w, z = 0..7) for the convergence check and the commit.
+template <int KIND>
+__device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t nxt, uint32_t p,
+                            float* sh, uint32_t* s_u32, float* s_f32, float* s_fk, GhcmInst in_single) {
+    constexpr int PX = kS3PX, PL = kS3Plane;
+    float* const t_s = sh;
+    float* const c_s = sh + kS3Box;
+    const int tid = threadIdx.x;
+    // The wave index, scalar: the octant, the face blocks and the owned row are wave-uniform.
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), L = tid & 63;
+    GhcmInst in = in_single;
+    if (a.n_inst != 1) in = uniform_inst(a.inst[a.tinst[g]]);
+    const uint32_t lt = g - in.tile_base;
+    const int tx = (int)(lt % in.tiles_x);
+    const int ty = (int)((lt / in.tiles_x) % in.tiles_y);
+    const int tz = (int)(lt / (in.tiles_x * in.tiles_y));
+    const int W = (int)in.w, H = (int)in.h, D = (int)in.d;
+    const int ox = tx * kS3X, oy = ty * kS3Y, oz = tz * kS3Z;
+    const int sx = min(kS3X, W - ox), sy = min(kS3Y, H - oy), sz = min(kS3Z, D - oz);
+    // Face keys of this visit (+INF: the neighbour across the face is not activated); read
+    // by the same threads at the end of the previous visit, written by atomics after the
+    // staging barrier.
+    if (tid < 6) s_fk[tid] = __builtin_inff();
+    stage_sweep3<KIND>(in, ox, oy, oz, tid, t_s, c_s);
+    __syncthreads();
+    float told[kS3Z], snap[kS3Z];
+#pragma unroll
+    for (int k = 0; k < kS3Z; ++k) {
+        told[k] = t_s[s3_box(L + 1, w + 1, k + 1)];
+        snap[k] = told[k];
+    }
+    const float eps = a.eps;
+    const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kS3HardRounds) ? kS3HardRounds : a.iter_cap;
+    char* const shb = reinterpret_cast<char*>(sh);
+    uint32_t rounds = 0;
+    bool unfinished = false;
+    for (;;) {
+        // Wave w runs octant w: bit 0 -x, bit 1 -y, bit 2 -z (the branch is wave-uniform).
+        switch (w) {
+            case 0: sweep3_round<1, 1, 1>(shb, L); break;
+            case 1: sweep3_round<-1, 1, 1>(shb, L); break;
+            case 2: sweep3_round<1, -1, 1>(shb, L); break;
+            case 3: sweep3_round<-1, -1, 1>(shb, L); break;
+            case 4: sweep3_round<1, 1, -1>(shb, L); break;
+            case 5: sweep3_round<-1, 1, -1>(shb, L); break;
+            case 6: sweep3_round<1, -1, -1>(shb, L); break;
+            default: sweep3_round<-1, -1, -1>(shb, L); break;
+        }
+        ++rounds;
+        __syncthreads();
+        bool chg = false;
+#pragma unroll
+        for (int k = 0; k < kS3Z; ++k) {
+            const float now = t_s[s3_box(L + 1, w + 1, k + 1)];
+            const float d = snap[k] - now;          // NaN (INF - INF) counts as no change
+            if (d > 0.0f && d >= eps) chg = true;
+            snap[k] = now;
+        }
+        const bool any = __ballot(chg) != 0ull;
+        if (L == 0) s_u32[w] = any ? 1u : 0u;
+        __syncthreads();
+        // No third barrier: a wave writes s_u32[w] again only after the next round's first
+        // barrier, which every wave reaches after reading it here.
+        uint32_t more = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) more |= s_u32[i];
+        if (!more) break;
+        if (rounds >= cap) { unfinished = true; break; }
+    }
+    // Commit improved cells (one writer per tile per wave): wave w writes row y = w of every
+    // z-plane, 64 contiguous cells each.
+    float tmin = __builtin_inff();
+    uint32_t improved = 0;
+#pragma unroll
+    for (int k = 0; k < kS3Z; ++k) {
+        if (L < sx && w < sy && k < sz) {
+            const float tn = t_s[s3_box(L + 1, w + 1, k + 1)];
+            tmin = fminf(tmin, tn);
+            if (tn < told[k]) {
+                const size_t idx = ((size_t)(oz + k) * (size_t)H + (size_t)(oy + w)) * (size_t)W + (size_t)(ox + L);
+                gmem(in.time)[idx] = tn;
+                ++improved;
+            }
+        }
+    }
+    // Face predictions: would the neighbour's cell across face s improve by >= eps from this
+    // tile's values (data/eikonal_ghcm_systolic.cl:8-41 in three dimensions)?  The 2176 face
+    // cells form 34 blocks of 64 (x-, x+, 8 of y-, 8 of y+, 8 of z-, 8 of z+); wave w takes
+    // blocks w, w+8, ..., so a block's face is wave-uniform.  The halo cell's neighbour further
+    // out is not staged and counts as +INF.
+    const int ntx = (int)in.tiles_x, nty = (int)in.tiles_y, ntz = (int)in.tiles_z;
+    for (int bi = w; bi < 34; bi += 8) {
+        int s, f;
+        if (bi < 2)       { s = bi == 0 ? 0 : 2; f = L; }
+        else if (bi < 18) { s = bi < 10 ? 1 : 3; f = ((bi - 2) & 7) * 64 + L; }
+        else              { s = bi < 26 ? 4 : 5; f = ((bi - 18) & 7) * 64 + L; }
+        const bool has_nb = s == 0 ? tx > 0 : s == 2 ? tx + 1 < ntx : s == 1 ? ty > 0
+                          : s == 3 ? ty + 1 < nty : s == 4 ? tz > 0 : tz + 1 < ntz;
+        if (!has_nb) continue;
+        int bx, by, bz;
+        bool in_grid;
+        if (s == 0 || s == 2) {
+            const int y = f & 7, z = f >> 3;
+            bx = s == 0 ? 0 : kS3X + 1; by = y + 1; bz = z + 1;
+            in_grid = y < sy && z < sz;
+        } else if (s == 1 || s == 3) {
+            const int x = f & 63, z = f >> 6;
+            bx = x + 1; by = s == 1 ? 0 : kS3Y + 1; bz = z + 1;
+            in_grid = x < sx && z < sz;
+        } else {
+            const int x = f & 63, y = f >> 6;
+            bx = x + 1; by = y + 1; bz = s == 4 ? 0 : kS3Z + 1;
+            in_grid = x < sx && y < sy;
+        }
+        float key = __builtin_inff();
+        if (in_grid) {
+            const int c = s3_box(bx, by, bz);
+            float ax, ay, az;
+            if (s == 0)      ax = t_s[c + 1];
+            else if (s == 2) ax = t_s[c - 1];
+            else             ax = fminf(t_s[c - 1], t_s[c + 1]);
+            if (s == 1)      ay = t_s[c + PX];
+            else if (s == 3) ay = t_s[c - PX];
+            else             ay = fminf(t_s[c - PX], t_s[c + PX]);
+            if (s == 4)      az = t_s[c + PL];
+            else if (s == 5) az = t_s[c - PL];
+            else             az = fminf(t_s[c - PL], t_s[c + PL]);
+            const float t0 = t_s[c];
+            const float tn = godunov3_sel(ax, ay, az, c_s[c]);
+            if (tn < t0 && t0 - tn >= eps) key = tn;
+        }
+        // Most blocks predict no improvement: only the others reduce their key and merge it.
+        if (__ballot(key < __builtin_inff()) == 0ull) continue;
+        key = wave_min(key);
+        if (L == 0)
+            __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(s_fk) + s, __float_as_uint(key),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    tmin = wave_min(tmin);
+    improved = wave_sum(improved);
+    if (L == 0) {
+        s_f32[w] = tmin;
+        s_u32[8 + w] = improved;
+    }
+    __syncthreads();
+    float tm = s_f32[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) tm = fminf(tm, s_f32[i]);
+    // Every activation after one barrier: the faces' neighbours (lanes 0-5 of wave 0) and,
+    // when the round cap stopped the tile, the tile itself (wave 1).
+    if (tid < 6) {
+        const float k = s_fk[tid];
+        if (k < __builtin_inff()) {
+            int dx, dy, dz;
+            side_step3(tid, dx, dy, dz);
+            const uint32_t nb = in.tile_base + (uint32_t)(tx + dx) +
+                                in.tiles_x * ((uint32_t)(ty + dy) + in.tiles_y * (uint32_t)(tz + dz));
+            activate(a, nb, wave, nxt, p, kMaskOpenAll, k);
+        }
+    }
+    if (tid == 64 && unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
+    if (tid == 128) {
+        uint32_t imp = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) imp += s_u32[8 + i];
+        a.tmin[g] = tm;
+        stat_add(&a.tstat[g], 1u);
+        stat_add(&a.tstat[(size_t)a.tiles + g], rounds);
+        stat_add(&a.tstat[2 * (size_t)a.tiles + g], imp);
+    }
+}
+
+// 2 work-groups per CU (the LDS boxes take 53.8 KB each): at most 128 VGPRs.
+template <int KIND>
+__global__ __launch_bounds__(kS3Threads) __attribute__((amdgpu_waves_per_eu(4))) void k_solve3d_sweep(SolveArgs a) {
+    __shared__ float sh[2 * kS3Box];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    __shared__ float s_fk[8];
+    __shared__ uint32_t s_prefix[kShards + 1];
+    const int tid = threadIdx.x;
+    const GhcmInst in0 = single_inst(a);
+    const GhcmCtrl* ctrl = a.ctrl;
+    publish_progress(a, ctrl);
+    const uint32_t done = ctrl->done;
+    const uint32_t nj = ctrl->n_jobs;
+    const uint32_t wave = ctrl->wave;
+    const uint32_t cur = ctrl->cur;
+    const uint32_t from_list = ctrl->jobs_from_list;
+    const uint32_t ncarry = ctrl->n_carry;
+    const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
+    // Work-groups past the jobs still take their share of the handed-off carry.
+    if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
+    const uint32_t p = wave & 1u;
+    if (tid <= (int)kShards) s_prefix[tid] = pv;
+    __syncthreads();
+    for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
+        const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
+        if (tid == 0) claim_tile(a, g, p);
+        sweep3_tile<KIND>(a, g, wave, cur ^ 1u, p, sh, s_u32, s_f32, s_fk, in0);
+        __syncthreads();
+    }
+    if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
+}
+
 // Seeds of 3-D solves: (x, y, z, instance) quadruples; T = 0 at the seed, and the tiles of
 // the seed and of its six neighbours are queued for the first wave with every cell open.
-__global__ void k_seed3d(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds) {
+__global__ void k_seed3d(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds, uint32_t tw, uint32_t th,
+                         uint32_t td) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_seeds) return;
     const uint32_t x = seeds[4 * s], y = seeds[4 * s + 1], z = seeds[4 * s + 2], ii = seeds[4 * s + 3];
@@ -2109,8 +2547,8 @@ __global__ void k_seed3d(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds) {
         if (d > 0) side_step3(d - 1, dx, dy, dz);
         const int px = (int)x + dx, py = (int)y + dy, pz = (int)z + dz;
         if (px < 0 || py < 0 || pz < 0 || px >= (int)in.w || py >= (int)in.h || pz >= (int)in.d) continue;
-        const uint32_t t = in.tile_base + (uint32_t)(px / kT3) +
-                           in.tiles_x * ((uint32_t)(py / kT3) + in.tiles_y * (uint32_t)(pz / kT3));
+        const uint32_t t = in.tile_base + (uint32_t)px / tw +
+                           in.tiles_x * ((uint32_t)py / th + in.tiles_y * ((uint32_t)pz / td));
         atomicMin(&a.key[t], 0u);
         atomicOr(&a.mask[t], kMaskOpenAll);
         const uint32_t old = atomicMax(&a.qwave[t], 1u);
@@ -2461,7 +2899,34 @@ static int occ_solve3d_t() {
     return nb > 0 ? nb : 1;
 }
 
-SolveKernel get_solve3d_kernel(int kind) {
+template <int KIND>
+static hipError_t launch_solve3d_sweep_t(const SolveArgs& a, uint32_t grid, hipStream_t s, hipEvent_t ev0,
+                                         hipEvent_t ev1) {
+    if (ev0 && ev1)
+        hipExtLaunchKernelGGL((k_solve3d_sweep<KIND>), dim3(grid), dim3(kS3Threads), 0, s, ev0, ev1, 0, a);
+    else
+        hipLaunchKernelGGL((k_solve3d_sweep<KIND>), dim3(grid), dim3(kS3Threads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int KIND>
+static int occ_solve3d_sweep_t() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve3d_sweep<KIND>, kS3Threads, 0) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+SolveKernel get_solve3d_kernel(int kind, bool sweep) {
+    if (sweep) {
+        switch (kind) {
+            case 0: return {&launch_solve3d_sweep_t<0>, &occ_solve3d_sweep_t<0>};
+            case 1: return {&launch_solve3d_sweep_t<1>, &occ_solve3d_sweep_t<1>};
+            case 2: return {&launch_solve3d_sweep_t<2>, &occ_solve3d_sweep_t<2>};
+            case 3: return {&launch_solve3d_sweep_t<3>, &occ_solve3d_sweep_t<3>};
+            default: return {nullptr, nullptr};
+        }
+    }
     switch (kind) {
         case 0: return {&launch_solve3d_t<0>, &occ_solve3d_t<0>};
         case 1: return {&launch_solve3d_t<1>, &occ_solve3d_t<1>};
@@ -2471,9 +2936,16 @@ SolveKernel get_solve3d_kernel(int kind) {
     }
 }
 
-hipError_t launch_seed3d(const SolveArgs& a, const uint32_t* seeds, uint32_t n_seeds, hipStream_t s) {
+void tile3d_shape(bool sweep, uint32_t& tw, uint32_t& th, uint32_t& td) {
+    tw = sweep ? (uint32_t)kS3X : (uint32_t)kT3;
+    th = sweep ? (uint32_t)kS3Y : (uint32_t)kT3;
+    td = sweep ? (uint32_t)kS3Z : (uint32_t)kT3;
+}
+
+hipError_t launch_seed3d(const SolveArgs& a, const uint32_t* seeds, uint32_t n_seeds, uint32_t tw,
+                         uint32_t th, uint32_t td, hipStream_t s) {
     if (n_seeds == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seed3d, dim3((n_seeds + 63) / 64), dim3(64), 0, s, a, seeds, n_seeds);
+    hipLaunchKernelGGL(k_seed3d, dim3((n_seeds + 63) / 64), dim3(64), 0, s, a, seeds, n_seeds, tw, th, td);
     return hipGetLastError();
 }
 

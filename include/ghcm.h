@@ -121,7 +121,7 @@ typedef struct ghcm_stats {
     uint32_t tile_w, tile_h;        /* tile shape used                                */
     uint32_t n_active;              /* n used                                         */
     uint32_t bytes_per_visit;       /* algorithmic HBM bytes of one tile visit        */
-    uint32_t reserved;
+    uint32_t tile_d;                /* tile depth (3-D solves; 1 otherwise)           */
     double   solve_kernel_us;       /* profile_kernels: summed tile-solve launch time */
     uint64_t solve_launches;        /* profile_kernels: tile-solve launches timed     */
 } ghcm_stats;
@@ -191,9 +191,11 @@ int ghcm_solve_device_noalloc(ghcm_ctx* ctx, const void* d_field, int field_kind
  * eikonal_solve<K>, src/eikonal.h:73-118 and data/eikonal_common.cl:221-285; the
  * reference declares K <= 3 but solves only 2-D grids) ----------------------------
  * `field` and the time are W*H*D cells, idx = x + W*(y + H*z); `seeds_xyz` holds
- * n_seeds (x, y, z) triples (host memory).  Tiles are 8x8x8 and the device scheduler
- * runs the wave loop: n_active, min_change, inner_iter_cap, waves_per_poll, max_waves
- * and launch_mode 2 (graph replay) apply, the tile shape and kernel fields do not, and
+ * n_seeds (x, y, z) triples (host memory).  kernel GHCM_KERNEL_SWEEP solves 64x8x8
+ * tiles with the eight octant sweeps (inner_iter_cap counts rounds of all eight), the
+ * other kernels 8x8x8 tiles with in-tile passes; the device scheduler runs the wave loop:
+ * n_active, min_change, inner_iter_cap, waves_per_poll, max_waves and launch_mode 2
+ * (graph replay) apply, the tile shape fields do not (stats report tile_w/h/d), and
  * the host scheduler returns GHCM_ERR_UNSUPPORTED.  Host-buffer and device-buffer forms
  * as for the 2-D solve. */
 int ghcm_solve3d(ghcm_ctx* ctx, const void* field, int field_kind, uint32_t w, uint32_t h,
