@@ -178,8 +178,9 @@ int ghcm_solve_device(ghcm_ctx* ctx, const void* d_field, int field_kind,
  * routine.  Its scheduler state lives in a static device array of 16 MiB per GPU, held
  * until the process ends, whose address ghcm_ctx_create looks up once per device; the
  * pinned records it uses are reserved by ghcm_ctx_create too.  Limits: the grid's tiles must
- * fit the array (about 300,000 tiles: a 4096^2 grid in 8x8 tiles, larger grids in larger
- * tiles; more is refused with GHCM_ERR_INVALID_ARG), at most 4,096 seeds, the device
+ * fit the array (52 B per tile: 322,602 tiles, a 4096^2 grid in 8x8 tiles; the sweep kernel
+ * adds 16 B per tile row for its edge columns: 15,590 tiles of 64x64, a 7,900^2 grid; more is
+ * refused with GHCM_ERR_INVALID_ARG), at most 4,096 seeds, the device
  * scheduler with launch_mode 0 or 4 and profile_kernels off (else GHCM_ERR_UNSUPPORTED).
  * Every call on a GPU shares that GPU's array: calls that overlap on one GPU, from any
  * context, race; each call starts by clearing what must start at zero. */
