@@ -16,13 +16,21 @@ every launch perturbs the loop, so `value` comes from the un-instrumented timed 
 the roofline from the same number of instrumented steps run right after them.
 
 Beside the headline the line carries:
-  batched    512 uneven instances in one device wave loop (autotuned over a small space)
-             vs the same solves launched one at a time, and the oracle FMM on all cores;
-  serving    the continuous-batching server on requests resident in HBM and on host
-             buffers: saturated solves/s vs one-at-a-time, p50/p99 latency under Poisson
-             arrivals at 80 % of the saturated rate;
-  precision  the 4096^2 field stored as FP32 cost and FP32/FP16/BF16 speed, L_inf/L1 vs
-             the FP64 FMM.
+  batched    512 uneven instances in one device wave loop (autotuned over tile shapes,
+             n, round caps and launch modes) vs the same solves launched one at a time,
+             the L_inf of every batched and one-at-a-time result vs the oracle's FP64 FMM,
+             and the oracle FMM on all cores;
+  serving    the continuous-batching server on 2,048 requests resident in HBM and 256 on
+             host buffers: saturated solves/s vs one-at-a-time, p50/p99 latency under
+             Poisson arrivals at 50 % and 80 % of the saturated rate, served results vs
+             the FP64 FMM, and the oracle FMM over the same fields on all cores;
+  precision  the 4096^2 smooth field and a 4096^2 ring field of contrast 1e4, each stored
+             as FP32 cost and FP32/FP16/BF16 speed, L_inf/L1 vs the FP64 FMM and the FP32
+             FIM;
+  volume     the 256^3 three-dimensional solve with the octant-sweep kernel (64x8x8
+             tiles), its roofline and parity vs the FP64 3-D FMM, the in-tile-pass kernel
+             (8x8x8 tiles) timed beside it, and the oracle's FP32 3-D FMM on one core.
+Every CPU baseline names the host CPU model and the cores it used.
 """
 from __future__ import annotations
 
