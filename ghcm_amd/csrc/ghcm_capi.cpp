@@ -916,6 +916,45 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                     if (!dirty[t]) { dirty[t] = 1; pq.push({0.0f, t}); }
                 }
             }
+        // launch_mode 2: a wave is one replay of a graph captured once -- the upload of n job
+        // slots, the tile launch over them and the read-back of their statuses -- instead of
+        // three enqueued operations.  Slots past the wave's jobs hold kNoTile, which the
+        // kernels skip.
+        const bool pq_graph = cfg.launch_mode == 2 && !cfg.profile_kernels;
+        uint32_t filled = 0;                    // slots [0, filled) may hold a stale tile id
+        if (pq_graph) {
+            for (uint32_t j = 0; j < n_sel; ++j) hj[j] = HostJob{kNoTile, 0u};
+            SolveArgs ga = a;
+            ga.n_host_jobs = n_sel;
+            std::vector<unsigned char> key(sizeof(SolveArgs) + sizeof(launch) + 2 * sizeof(void*) + 1);
+            unsigned char* kp = key.data();
+            std::memcpy(kp, &ga, sizeof(SolveArgs));
+            std::memcpy(kp + sizeof(SolveArgs), &launch, sizeof(launch));
+            std::memcpy(kp + sizeof(SolveArgs) + sizeof(launch), &hj, sizeof(void*));
+            std::memcpy(kp + sizeof(SolveArgs) + sizeof(launch) + sizeof(void*), &hst, sizeof(void*));
+            kp[key.size() - 1] = 0x9A;          // tells host-queue graphs from device-loop ones
+            if (!ctx->gexec || key != ctx->gkey) {
+                if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+                ctx->gexec = nullptr;
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                hipError_t ce = hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * n_sel, hipMemcpyHostToDevice, st);
+                if (ce == hipSuccess) ce = launch(ga, n_sel, st, nullptr, nullptr);
+                if (ce == hipSuccess)
+                    ce = hipMemcpyAsync(hst, ga.status, sizeof(HostStatus) * n_sel, hipMemcpyDeviceToHost, st);
+                hipError_t ee = hipStreamEndCapture(st, &graph);
+                if (ce != hipSuccess || ee != hipSuccess)
+                    return fail(GHCM_ERR_HIP, "graph capture failed: %s",
+                                hipGetErrorString(ce != hipSuccess ? ce : ee));
+                hipError_t ie = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    ctx->gexec = nullptr;
+                    return fail(GHCM_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+                }
+                ctx->gkey = key;
+            }
+        }
         while (!pq.empty()) {
             if (waves_done >= max_waves) break;
             uint32_t nj = 0;
@@ -928,13 +967,19 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
                 dirty[id] = 0;   // cleared before results, so same-wave neighbours re-queue
                 ++nj;
             }
-            a.n_host_jobs = nj;
-            HIP_TRY(hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * nj, hipMemcpyHostToDevice, st));
-            if (cfg.profile_kernels)
-                HIP_TRY(launch(a, nj, st, ctx->ev_prof[0], ctx->ev_prof[1]));
-            else
-                HIP_TRY(launch(a, nj, st, nullptr, nullptr));
-            HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
+            if (pq_graph) {
+                for (uint32_t j = nj; j < filled; ++j) hj[j] = HostJob{kNoTile, 0u};
+                filled = nj;
+                HIP_TRY(hipGraphLaunch(ctx->gexec, st));
+            } else {
+                a.n_host_jobs = nj;
+                HIP_TRY(hipMemcpyAsync(ctx->jobs.p, hj, sizeof(HostJob) * nj, hipMemcpyHostToDevice, st));
+                if (cfg.profile_kernels)
+                    HIP_TRY(launch(a, nj, st, ctx->ev_prof[0], ctx->ev_prof[1]));
+                else
+                    HIP_TRY(launch(a, nj, st, nullptr, nullptr));
+                HIP_TRY(hipMemcpyAsync(hst, a.status, sizeof(HostStatus) * nj, hipMemcpyDeviceToHost, st));
+            }
             HIP_TRY(ghcm_capi::spin_sync(st, ctx->ev_sync));
             if (cfg.profile_kernels) {
                 float ms = 0.0f;

@@ -657,6 +657,8 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         const uint32_t j = blockIdx.x;
         if (j >= a.n_host_jobs) return;
         const HostJob job = a.host_jobs[j];
+        // A graph-replayed wave launches n work-groups; its unused job slots hold kNoTile.
+        if (job.id == kNoTile) return;
         process_tile<TW, TH, KIND, MODE, true>(a, job.id, job.border, 0u, 0u, 0u, j,
                                                t_s, c_s, open_s, add_s);
     } else {
@@ -1015,7 +1017,9 @@ void k_solve_sweep(SolveArgs a) {
     if constexpr (HOST) {
         const uint32_t j = blockIdx.x;
         if (j >= a.n_host_jobs) return;
-        sweep_tile<TW, TH, KIND, true>(a, a.host_jobs[j].id, 0u, 0u, 0u, j, sh, s_u32, s_f32, in0);
+        const uint32_t id = a.host_jobs[j].id;
+        if (id == kNoTile) return;          // an unused slot of a graph-replayed wave
+        sweep_tile<TW, TH, KIND, true>(a, id, 0u, 0u, 0u, j, sh, s_u32, s_f32, in0);
     } else {
         const GhcmCtrl* ctrl = a.ctrl;
         publish_progress(a, ctrl);

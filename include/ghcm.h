@@ -98,7 +98,9 @@ typedef struct ghcm_config {
                                        wave (default), 1 = one persistent cooperative
                                        launch runs every wave where built (sweep kernel;
                                        its grid barrier pays agent-scope fences),
-                                       2 = chunks of waves replayed as a HIP graph,
+                                       2 = chunks of waves replayed as a HIP graph (host
+                                       scheduler: each wave's job upload, launch and
+                                       status read-back replayed as one graph),
                                        3 = one launch per wave: the tile-solve kernel's
                                        last work-group runs the wave boundary (sweep),
                                        4 = boundary-free waves (sweep kernel): one

@@ -62,7 +62,8 @@ def test_rings256_every_config(rings256, kernel, tile, sched):
 @pytest.mark.parametrize("tile", SWEEP_TILES)
 @pytest.mark.parametrize("sched,launch_mode", [(g.SCHED_DEVICE_TOPN, 0), (g.SCHED_DEVICE_TOPN, 1),
                                                (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
-                                               (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_HOST_PQ, 0)])
+                                               (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_HOST_PQ, 0),
+                                               (g.SCHED_HOST_PQ, 2)])
 @pytest.mark.parametrize("cap", [0, 1])
 def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     cost, seeds, ref32, ref64 = rings256
@@ -73,6 +74,38 @@ def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     assert st["converged"] == 1
     check(t, ref32, seeds, cost, label="sweep vs fmm32")
     check(t, ref64.astype(np.float64), seeds, label="sweep vs fmm64 fixture")
+
+
+@pytest.mark.parametrize("kernel,tile", [(g.KERNEL_SYSTOLIC, (8, 8)), (g.KERNEL_FIM, (16, 16)),
+                                         (g.KERNEL_SWEEP, (32, 32))])
+def test_host_queue_graph_waves_equal_enqueued_waves(rings256, kernel, tile):
+    """The host priority queue with each wave replayed as one captured graph (job upload,
+    tile launch over n slots, status read-back) pops the same jobs in the same order as
+    with three enqueued operations per wave, so with the one-wave tile kernels (systolic,
+    FIM) its times are identical; the sweep kernel's four waves interleave their LDS
+    updates freely, so it is held to the oracle.  A second solve of another size
+    re-captures, and a third of the first size replays again."""
+    cost, seeds, ref32, _ = rings256
+    small = np.ascontiguousarray(cost[:200, :136])
+    sseeds = np.array([[17, 150]], np.uint32)
+    print(f"case host-queue graph waves kernel={kernel} tile={tile}", flush=True)
+    out = {}
+    for mode in (0, 2):
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=kernel, scheduler=g.SCHED_HOST_PQ,
+                       launch_mode=mode, n_active=48) as ctx:
+            a, sa = ctx.solve(cost, seeds)
+            b, _ = ctx.solve(small, sseeds)
+            c, sc = ctx.solve(cost, seeds)
+        assert sa["converged"] == 1 and sa["waves"] > 10
+        out[mode] = (a, b, c, sa["waves"], sa["tile_visits"])
+    if kernel != g.KERNEL_SWEEP:
+        for k in range(3):
+            assert np.array_equal(out[0][k], out[2][k])
+        assert out[0][3:] == out[2][3:]
+        assert np.array_equal(out[2][0], out[2][2])
+    check(out[2][0], ref32, seeds, cost, label="host-queue graph vs fmm32")
+    check(out[2][2], ref32, seeds, cost, label="host-queue graph, replayed again")
+    check(out[2][1], oracle.fmm_f32(small, sseeds), sseeds, small, label="host-queue graph, second size")
 
 
 @pytest.mark.parametrize("sweep", [False, True])
