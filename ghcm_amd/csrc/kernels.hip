@@ -1884,11 +1884,11 @@ __device__ __forceinline__ float godunov3_raw(float a, float b, float c, float f
 // operations in the same order: the result equals godunov3_raw's bit for bit.  The inputs are
 // non-negative or +INF, so the sort uses integer minimum and maximum.
 __device__ __forceinline__ float godunov3_sel(float a, float b, float c, float f) {
-    // Sorted by three-operand minimum, median and maximum of the bit patterns.
-    const uint32_t ua = __float_as_uint(a), ub = __float_as_uint(b), uc = __float_as_uint(c);
-    const float m1 = __uint_as_float(min(min(ua, ub), uc));
-    const float m3 = __uint_as_float(max(max(ua, ub), uc));
-    const float m2 = __uint_as_float(max(min(ua, ub), min(max(ua, ub), uc)));
+    // Sorted by three-operand minimum, median and maximum (v_min3/v_med3/v_max3: the values
+    // are non-negative or +INF, never NaN, so the float forms order them exactly).
+    const float m1 = fminf(fminf(a, b), c);
+    const float m3 = fmaxf(fmaxf(a, b), c);
+    const float m2 = __builtin_amdgcn_fmed3f(a, b, c);
     const float t1 = m1 + f;
     // godunov_raw(m1, m2, m1, f) with its |m1 - m2| formed as m2 - m1 (the same value: the
     // inputs are sorted), shared with the three-axis form.
@@ -2208,11 +2208,13 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
         return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
     };
     const float inf = __builtin_inff();
-    float prev = inf;
+    // The previous step's result of this lane (upwind x), of the y-upwind lane (DPP row shift)
+    // and of the z-upwind lane (ds_bpermute).  Both moves are issued as soon as the result
+    // exists, ahead of the next step's LDS reads, so waiting for the permute does not wait for
+    // them.  The DPP destination is carried from step to step: lanes without a source lane keep
+    // their old value, and every such lane is one the y-halo mask replaces.
+    float prev = inf, yv = inf, zv = inf;
     auto step = [&](const Nb& n) {
-        const float yv = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(inf), __float_as_int(prev),
-                                                                    kDppY, 0xF, 0xF, false));
-        const float zv = __int_as_float(__builtin_amdgcn_ds_bpermute(zsrc, __float_as_int(prev)));
         const float ax = umin(n.ux, n.first ? inf : prev);
         const float ay = umin(n.uy, __builtin_amdgcn_inverse_ballot_w64(kYHalo) ? inf : yv);
         const float az = umin(n.uz, __builtin_amdgcn_inverse_ballot_w64(kZHalo) ? inf : zv);
@@ -2220,6 +2222,9 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
         __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * kCell), __float_as_uint(t),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         prev = t;
+        zv = __int_as_float(__builtin_amdgcn_ds_bpermute(zsrc, __float_as_int(t)));
+        yv = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(yv), __float_as_int(t), kDppY, 0xF, 0xF,
+                                                        false));
     };
     // Loads of step s+1 are issued before step s is computed (two register sets): their cells
     // are upper bounds whatever the interleaving, and the values step s produces reach step
@@ -2231,9 +2236,11 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
         load(B, st + 1);
         __builtin_amdgcn_sched_barrier(0);
         step(A);
+        __builtin_amdgcn_sched_barrier(0);
         load(A, st + 2);
         __builtin_amdgcn_sched_barrier(0);
         step(B);
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
