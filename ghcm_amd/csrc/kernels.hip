@@ -100,16 +100,36 @@ __device__ __forceinline__ float godunov(float ax, float ay, float c, float t) {
     return fminf(t, godunov_raw(ax, ay, fminf(ax, ay), c));
 }
 
+// Wave reductions: a DPP butterfly inside each row of 16 lanes (quad_perm [1,0,3,2] and
+// [2,3,0,1], then row_half_mirror and row_mirror, after which every lane of a row holds its
+// row's result), then the four rows' results read into scalar registers and combined there.
+// No LDS round trip (a __shfl_xor butterfly is six dependent ds_bpermute).  Every lane must be
+// active, as at every call site; the result is wave-uniform.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_reduce_u32(uint32_t v, Op op) {
+    v = op(v, dpp_u32<0xB1>(v));
+    v = op(v, dpp_u32<0x4E>(v));
+    v = op(v, dpp_u32<0x141>(v));
+    v = op(v, dpp_u32<0x140>(v));
+    const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
+// Minimum of times or keys: non-negative or +INF, so their bit patterns order like the values.
 __device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    return v;
+    return __uint_as_float(wave_reduce_u32(__float_as_uint(v), [](uint32_t x, uint32_t y) { return min(x, y); }));
 }
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return wave_reduce_u32(v, [](uint32_t x, uint32_t y) { return x + y; });
 }
 
 // Stores to mapped host memory: system scope, relaxed (each word is read on its own).
