@@ -2449,45 +2449,40 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
     // cells form 34 blocks of 64 (x-, x+, 8 of y-, 8 of y+, 8 of z-, 8 of z+); wave w takes
     // blocks w, w+8, ..., so a block's face is wave-uniform.  The halo cell's neighbour further
     // out is not staged and counts as +INF.
+    // One code path for every face: the block's face gives (wave-uniform) the halo cell's box
+    // word as base + u * su + v * sv for in-face coordinates (u, v) = (f & mu, f >> shv), the
+    // word offset of its one staged neighbour inside the tile, and the offsets of its two
+    // lateral pairs; godunov3_sel sorts its three axis values, so their order is free.
     const int ntx = (int)in.tiles_x, nty = (int)in.tiles_y, ntz = (int)in.tiles_z;
+    auto umin = [](float u, float v) { return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v))); };
     for (int bi = w; bi < 34; bi += 8) {
-        int s, f;
-        if (bi < 2)       { s = bi == 0 ? 0 : 2; f = L; }
-        else if (bi < 18) { s = bi < 10 ? 1 : 3; f = ((bi - 2) & 7) * 64 + L; }
-        else              { s = bi < 26 ? 4 : 5; f = ((bi - 18) & 7) * 64 + L; }
+        int s, blk;
+        if (bi < 2)       { s = bi == 0 ? 0 : 2; blk = 0; }
+        else if (bi < 18) { s = bi < 10 ? 1 : 3; blk = (bi - 2) & 7; }
+        else              { s = bi < 26 ? 4 : 5; blk = (bi - 18) & 7; }
         const bool has_nb = s == 0 ? tx > 0 : s == 2 ? tx + 1 < ntx : s == 1 ? ty > 0
                           : s == 3 ? ty + 1 < nty : s == 4 ? tz > 0 : tz + 1 < ntz;
         if (!has_nb) continue;
-        int bx, by, bz;
-        bool in_grid;
-        if (s == 0 || s == 2) {
-            const int y = f & 7, z = f >> 3;
-            bx = s == 0 ? 0 : kS3X + 1; by = y + 1; bz = z + 1;
-            in_grid = y < sy && z < sz;
-        } else if (s == 1 || s == 3) {
-            const int x = f & 63, z = f >> 6;
-            bx = x + 1; by = s == 1 ? 0 : kS3Y + 1; bz = z + 1;
-            in_grid = x < sx && z < sz;
-        } else {
-            const int x = f & 63, y = f >> 6;
-            bx = x + 1; by = y + 1; bz = s == 4 ? 0 : kS3Z + 1;
-            in_grid = x < sx && y < sy;
-        }
+        const int f = blk * 64 + L;
+        const bool xf = s == 0 || s == 2, yf = s == 1 || s == 3;
+        const int mu = xf ? 7 : 63, shv = xf ? 3 : 6;
+        const int su = xf ? PX : 1, sv = xf ? PL : yf ? PL : PX;
+        const int eu = xf ? sy : sx, ev = xf ? sz : yf ? sz : sy;
+        // the face plane's box coordinate along its normal, and the unit step inwards
+        const int nrm = s == 0 ? 0 : s == 2 ? kS3X + 1 : s == 1 ? 0 : s == 3 ? kS3Y + 1 : s == 4 ? 0 : kS3Z + 1;
+        const int nst = xf ? 1 : yf ? PX : PL;
+        const int base = xf ? nrm + PX + PL : yf ? 1 + nrm * PX + PL : 1 + PX + nrm * PL;
+        const int in_off = (s == 0 || s == 1 || s == 4) ? nst : -nst;
+        const int l1 = xf ? PX : 1, l2 = xf ? PL : yf ? PL : PX;
+        const int u = f & mu, v = f >> shv;
         float key = __builtin_inff();
-        if (in_grid) {
-            const int c = s3_box(bx, by, bz);
-            float ax, ay, az;
-            if (s == 0)      ax = t_s[c + 1];
-            else if (s == 2) ax = t_s[c - 1];
-            else             ax = fminf(t_s[c - 1], t_s[c + 1]);
-            if (s == 1)      ay = t_s[c + PX];
-            else if (s == 3) ay = t_s[c - PX];
-            else             ay = fminf(t_s[c - PX], t_s[c + PX]);
-            if (s == 4)      az = t_s[c + PL];
-            else if (s == 5) az = t_s[c - PL];
-            else             az = fminf(t_s[c - PL], t_s[c + PL]);
+        if (u < eu && v < ev) {
+            const int c = base + u * su + v * sv;
+            const float a_in = t_s[c + in_off];
+            const float a_1 = umin(t_s[c - l1], t_s[c + l1]);
+            const float a_2 = umin(t_s[c - l2], t_s[c + l2]);
             const float t0 = t_s[c];
-            const float tn = godunov3_sel(ax, ay, az, c_s[c]);
+            const float tn = godunov3_sel(a_in, a_1, a_2, c_s[c]);
             if (tn < t0 && t0 - tn >= eps) key = tn;
         }
         // Most blocks predict no improvement: only the others reduce their key and merge it.
