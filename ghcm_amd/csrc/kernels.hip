@@ -694,8 +694,13 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         const uint32_t from_list = ctrl->jobs_from_list;
         const uint32_t ncarry = ctrl->n_carry;
         static_assert(kShards < 64, "one lane per prefix entry");
-        const uint32_t pv = lane <= (int)kShards ? ctrl->prefix[lane] : 0u;
+        // Every lane reads a valid word (no branch around the load).
+        const uint32_t pv = ctrl->prefix[min(lane, (int)kShards)];
         // Work-groups past the jobs still take their share of the handed-off carry.
+        // A compiler barrier: every load above is issued before the done test (without it the
+        // short-circuit test lets the compiler sink the later loads below it, one memory round
+        // trip each).
+        __asm__ volatile("" ::: "memory");
         if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
         const uint32_t p = wave & 1u;
         if (lane <= (int)kShards) s_prefix[lane] = pv;
@@ -1056,6 +1061,10 @@ void k_solve_sweep(SolveArgs a) {
         const uint32_t pr = ctrl->prefix[lane & (int)(kShards - 1)];
         const uint32_t pv = lane < (int)kShards ? pr : 0xFFFFFFFFu;
         // Work-groups past the jobs still take their share of the handed-off carry.
+        // A compiler barrier: every load above is issued before the done test (without it the
+        // short-circuit test lets the compiler sink the later loads below it, one memory round
+        // trip each).
+        __asm__ volatile("" ::: "memory");
         if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
         const uint32_t p = wave & 1u;
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
@@ -2131,8 +2140,13 @@ __global__ __launch_bounds__(kT3Threads) __attribute__((amdgpu_waves_per_eu(8)))
     const uint32_t cur = ctrl->cur;
     const uint32_t from_list = ctrl->jobs_from_list;
     const uint32_t ncarry = ctrl->n_carry;
-    const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
+    // Every thread reads a valid word (no branch around the load).
+    const uint32_t pv = ctrl->prefix[min(tid, (int)kShards)];
     // Work-groups past the jobs still take their share of the handed-off carry.
+    // A compiler barrier: every load above is issued before the done test (without it the
+    // short-circuit test lets the compiler sink the later loads below it, one memory round
+    // trip each).
+    __asm__ volatile("" ::: "memory");
     if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
     const uint32_t p = wave & 1u;
     if (tid <= (int)kShards) s_prefix[tid] = pv;
@@ -2544,8 +2558,13 @@ __global__ __launch_bounds__(kS3Threads) __attribute__((amdgpu_waves_per_eu(4)))
     const uint32_t cur = ctrl->cur;
     const uint32_t from_list = ctrl->jobs_from_list;
     const uint32_t ncarry = ctrl->n_carry;
-    const uint32_t pv = tid <= (int)kShards ? ctrl->prefix[tid] : 0u;
+    // Every thread reads a valid word (no branch around the load).
+    const uint32_t pv = ctrl->prefix[min(tid, (int)kShards)];
     // Work-groups past the jobs still take their share of the handed-off carry.
+    // A compiler barrier: every load above is issued before the done test (without it the
+    // short-circuit test lets the compiler sink the later loads below it, one memory round
+    // trip each).
+    __asm__ volatile("" ::: "memory");
     if (done || (blockIdx.x >= nj && ncarry == 0u)) return;
     const uint32_t p = wave & 1u;
     if (tid <= (int)kShards) s_prefix[tid] = pv;
