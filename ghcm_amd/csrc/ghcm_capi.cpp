@@ -521,8 +521,9 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     HIP_TRY(hipSetDevice(ctx->ordinal));
     hipStream_t st = ctx->stream;
     const uint32_t shard_cap = (T + kShards - 1) / kShards;
-    // The sweep kernel stages its halo columns from edge columns kept for every tile.
-    const uint32_t edge_h = (!three_d && cfg.kernel == GHCM_KERNEL_SWEEP) ? th : 0u;
+    // The sweep kernels stage their halo columns from edge columns kept for every tile
+    // (th * td rows each; a 3-D sweep tile's 64 rows).
+    const uint32_t edge_h = (cfg.kernel == GHCM_KERNEL_SWEEP) ? th * td : 0u;
     float *d_edge = nullptr, *d_cedge = nullptr;
     // Device scheduler state: the context's grow-only buffers, or the scratch array.
     GhcmInst* d_inst = nullptr;
@@ -636,7 +637,7 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     // the whole solve issues no runtime copy.
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
     HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st));
-    HIP_TRY(launch_edges_init(a, 0, T, tw, th, kind, st));
+    HIP_TRY(launch_edges_init(a, 0, T, tw, th, td, kind, st));
     if (three_d)
         HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, td, st));
     else

@@ -58,7 +58,7 @@ hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // Edge columns of tiles [t0, t0 + count) after the time fields were set to +INF (sweep
 // kernel; nothing when a.edge is null).  Launched before the seeds.
 hipError_t launch_edges_init(const SolveArgs& a, uint32_t t0, uint32_t count, uint32_t tw,
-                             uint32_t th, int kind, hipStream_t s);
+                             uint32_t th, uint32_t td, int kind, hipStream_t s);
 hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s);
 hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out,
                               hipStream_t s);

@@ -1625,20 +1625,24 @@ __device__ __forceinline__ void seed_edges(const SolveArgs& a, const GhcmInst& i
 // and the cost of the same cells (+INF past the grid).  One work-group of 2 * th threads per
 // tile, one cell each.
 template <int KIND>
-__global__ void k_edges_init(SolveArgs a, uint32_t t0, uint32_t count, uint32_t tw, uint32_t th) {
+__global__ void k_edges_init(SolveArgs a, uint32_t t0, uint32_t count, uint32_t tw, uint32_t th, uint32_t td) {
+    // An edge column has th * td rows: row r is (y, z) = (r mod th, r / th) of the tile (td = 1
+    // in two dimensions).
+    const uint32_t eh = th * td;
     for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
         const uint32_t g = t0 + k;
         const uint32_t ii = a.n_inst == 1 ? 0u : a.tinst[g];
         const GhcmInst in = a.inst[ii];
-        int tx, ty;
-        tile_xy(g - in.tile_base, in.tiles_x, tx, ty);
-        const uint32_t j = threadIdx.x, right = j >= th ? 1u : 0u, r = j - right * th;
-        const uint32_t x = (uint32_t)tx * tw + (right ? tw - 1 : 0u), y = (uint32_t)ty * th + r;
+        const uint32_t lt = g - in.tile_base;
+        const uint32_t tx = lt % in.tiles_x, ty = (lt / in.tiles_x) % in.tiles_y;
+        const uint32_t tz = lt / (in.tiles_x * in.tiles_y);
+        const uint32_t j = threadIdx.x, right = j >= eh ? 1u : 0u, r = j - right * eh;
+        const uint32_t x = tx * tw + (right ? tw - 1 : 0u), y = ty * th + r % th, z = tz * td + r / th;
         float c = __builtin_inff();
-        if (x < in.w && y < in.h)
+        if (x < in.w && y < in.h && z < in.d)
             c = load_cost_at<KIND>(static_cast<const char*>(in.field) +
-                                   ((size_t)y * in.w + x) * field_bytes<KIND>());
-        const size_t o = (size_t)g * 2 * th + j;
+                                   (((size_t)z * in.h + y) * in.w + x) * field_bytes<KIND>());
+        const size_t o = (size_t)g * 2 * eh + j;
         gmem(a.edge)[o] = __builtin_inff();
         gmem(const_cast<float*>(a.cedge))[o] = c;
     }
@@ -2278,17 +2282,19 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
     }
 }
 
-// Stage the (kS3X + 2) x 10 x 10 box of 3-D sweep tile (ox, oy, oz): the time box without its
-// eight corners, the cost box at the tile and its six faces (edges and corners are never read).
-// Rows of the box along x are read as aligned runs of 64 cells (the tile's x range), one run
-// per wave-uniform row job; the x = -1 and x = 64 words of the rows are read one per thread.
-// Out-of-grid cells read +INF.  Every global load is issued before the first LDS store.
+// Stage the (kS3X + 2) x 10 x 10 box of 3-D sweep tile g at (ox, oy, oz): the time box, the
+// cost box at the tile and its six faces (edges and corners are never read).  Rows of the box
+// along x are read as aligned runs of 64 cells (the tile's x range), one run per wave-uniform
+// row job.  The x = -1 and x = 64 words of the tile's 64 rows are the neighbours' edge columns
+// (a.edge, a.cedge: 64 contiguous floats per side, row r = (r mod 8, r / 8)); those of the 36
+// halo rows (time only) are read one per thread.  Out-of-grid cells read +INF.  Every global
+// load is issued before the first LDS store.
 template <int KIND>
-__device__ __forceinline__ void stage_sweep3(const GhcmInst& in, int ox, int oy, int oz, int tid,
-                                             float* t_s, float* c_s) {
+__device__ __forceinline__ void stage_sweep3(const SolveArgs& a, const GhcmInst& in, uint32_t g, int tx,
+                                             int ox, int oy, int oz, int tid, float* t_s, float* c_s) {
     constexpr int kRowJobs = 64 + 64 + 36 + 32;        // time, cost interior; time, cost halo rows
     constexpr int NR = (kRowJobs + 7) / 8;
-    constexpr int kHaloWords = 2 * 100 + 2 * 64;       // time: every box row; cost: interior rows
+    constexpr int kHaloWords = 4 * 64 + 2 * 36;        // edge columns (time, cost); halo rows' words
     // The wave index stays a vector value here: as a scalar, the 25 row jobs' addresses
     // exceed the scalar registers and spill.
     const int w = tid >> 6, L = tid & 63;
@@ -2342,24 +2348,30 @@ __device__ __forceinline__ void stage_sweep3(const GhcmInst& in, int ox, int oy,
     float hv = __builtin_inff();
     int hq = -1;
     bool hcost = false;
-    if (tid < kHaloWords) {
-        int by, bz, bx;
-        if (tid < 200) {
-            const int q = tid >> 1;
-            by = q % 10;
-            bz = q / 10;
-        } else {
-            const int q = (tid - 200) >> 1;
-            by = (q & 7) + 1;
-            bz = (q >> 3) + 1;
-            hcost = true;
+    if (tid < 4 * 64) {
+        // Edge columns: tid 0-127 time, 128-255 cost; the left halo column is the left
+        // neighbour's last column, the right one the right neighbour's first column.
+        const int r = tid & 63, right = (tid >> 6) & 1;
+        hcost = tid >= 128;
+        const int y = r & 7, z = r >> 3;
+        hq = s3_box(right ? kS3X + 1 : 0, y + 1, z + 1);
+        const bool has = right ? ox + kS3X < W : tx > 0;
+        if (has && oy + y < H && oz + z < D) {
+            const size_t o = (right ? (size_t)g + 1 : (size_t)g - 1) * 2 * 64 + (right ? 0 : 64) + (size_t)r;
+            hv = *gmem(hcost ? a.cedge + o : a.edge + o);
         }
-        bx = (tid & 1) ? kS3X + 1 : 0;
+    } else if (tid < kHaloWords) {
+        // The x = -1 and x = 64 time words of the 36 halo rows (the y-halo rows of the tile
+        // planes, then both z-halo planes whole).
+        int q = (tid - 4 * 64) >> 1, by, bz;
+        if (q < 16) { by = (q & 1) ? kS3Y + 1 : 0; bz = (q >> 1) + 1; }
+        else        { q -= 16; bz = q >= 10 ? kS3Z + 1 : 0; by = q % 10; }
+        const int bx = (tid & 1) ? kS3X + 1 : 0;
         hq = s3_box(bx, by, bz);
         const int gx = ox + bx - 1, gy = oy + by - 1, gz = oz + bz - 1;
         if (gx >= 0 && gy >= 0 && gz >= 0 && gx < W && gy < H && gz < D) {
             const size_t idx = ((size_t)gz * (size_t)H + (size_t)gy) * (size_t)W + (size_t)gx;
-            hv = hcost ? load_cost_at<KIND>(fb + idx * fbytes) : *gmem(reinterpret_cast<const float*>(tb + idx * 4));
+            hv = *gmem(reinterpret_cast<const float*>(tb + idx * 4));
         }
     }
 #pragma unroll
@@ -2396,7 +2408,7 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
     // by the same threads at the end of the previous visit, written by atomics after the
     // staging barrier.
     if (tid < 6) s_fk[tid] = __builtin_inff();
-    stage_sweep3<KIND>(in, ox, oy, oz, tid, t_s, c_s);
+    stage_sweep3<KIND>(a, in, g, tx, ox, oy, oz, tid, t_s, c_s);
     __syncthreads();
     float told[kS3Z], snap[kS3Z];
 #pragma unroll
@@ -2457,6 +2469,14 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
                 ++improved;
             }
         }
+    }
+    // Edge columns from the final box: wave 0 the first column, wave 1 the last one when the
+    // tile is full width (only then has it a right neighbour that reads it); lane L is row
+    // (L mod 8, L / 8), and rows past the grid are never read and not written.
+    if (w < 2 && (w == 0 || sx == kS3X)) {
+        const int y = L & 7, z = L >> 3;
+        if (y < sy && z < sz)
+            gmem(a.edge)[(size_t)g * 2 * 64 + (size_t)w * 64 + L] = t_s[s3_box(w == 0 ? 1 : kS3X, y + 1, z + 1)];
     }
     // Face predictions: would the neighbour's cell across face s improve by >= eps from this
     // tile's values (data/eikonal_ghcm_systolic.cl:8-41 in three dimensions)?  The 2176 face
@@ -2587,6 +2607,13 @@ __global__ void k_seed3d(SolveArgs a, const uint32_t* seeds, uint32_t n_seeds, u
     const uint32_t x = seeds[4 * s], y = seeds[4 * s + 1], z = seeds[4 * s + 2], ii = seeds[4 * s + 3];
     const GhcmInst in = a.inst[ii];
     gmem(in.time)[((size_t)z * in.h + y) * in.w + x] = 0.0f;
+    if (a.edge) {
+        // A seed on its tile's first or last column also sets that tile's edge column.
+        const uint32_t t = in.tile_base + x / tw + in.tiles_x * (y / th + in.tiles_y * (z / td));
+        const uint32_t eh = th * td, r = y % th + th * (z % td), cx = x % tw;
+        if (cx == 0) gmem(a.edge)[(size_t)t * 2 * eh + r] = 0.0f;
+        if (cx == tw - 1) gmem(a.edge)[(size_t)t * 2 * eh + eh + r] = 0.0f;
+    }
     for (int d = 0; d < 7; ++d) {
         int dx = 0, dy = 0, dz = 0;
         if (d > 0) side_step3(d - 1, dx, dy, dz);
@@ -2832,15 +2859,16 @@ hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s) {
 }
 
 hipError_t launch_edges_init(const SolveArgs& a, uint32_t t0, uint32_t count, uint32_t tw,
-                             uint32_t th, int kind, hipStream_t s) {
+                             uint32_t th, uint32_t td, int kind, hipStream_t s) {
     if (!a.edge || count == 0) return hipSuccess;
+    if (2 * th * td > 1024 || th * td != a.edge_h) return hipErrorInvalidValue;
     const unsigned blocks = count < 65536u ? count : 65536u;
-    const dim3 grid(blocks), block(2 * th);
+    const dim3 grid(blocks), block(2 * th * td);
     switch (kind) {
-        case 0: hipLaunchKernelGGL(k_edges_init<0>, grid, block, 0, s, a, t0, count, tw, th); break;
-        case 1: hipLaunchKernelGGL(k_edges_init<1>, grid, block, 0, s, a, t0, count, tw, th); break;
-        case 2: hipLaunchKernelGGL(k_edges_init<2>, grid, block, 0, s, a, t0, count, tw, th); break;
-        default: hipLaunchKernelGGL(k_edges_init<3>, grid, block, 0, s, a, t0, count, tw, th); break;
+        case 0: hipLaunchKernelGGL(k_edges_init<0>, grid, block, 0, s, a, t0, count, tw, th, td); break;
+        case 1: hipLaunchKernelGGL(k_edges_init<1>, grid, block, 0, s, a, t0, count, tw, th, td); break;
+        case 2: hipLaunchKernelGGL(k_edges_init<2>, grid, block, 0, s, a, t0, count, tw, th, td); break;
+        default: hipLaunchKernelGGL(k_edges_init<3>, grid, block, 0, s, a, t0, count, tw, th, td); break;
     }
     return hipGetLastError();
 }
