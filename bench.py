@@ -27,6 +27,8 @@ Beside the headline the line carries:
   precision  the 4096^2 smooth field and a 4096^2 ring field of contrast 1e4, each stored
              as FP32 cost and FP32/FP16/BF16 speed, L_inf/L1 vs the FP64 FMM and the FP32
              FIM;
+  reference_kernels  the 4096^2 solve with the systolic (8x8, 32x8) and FIM (16x16) tile
+             kernels at the reference's default wave size, each with its roofline;
   volume     the 256^3 three-dimensional solve with the octant-sweep kernel (64x8x8
              tiles), its roofline and parity vs the FP64 3-D FMM, the in-tile-pass kernel
              (8x8x8 tiles) timed beside it, and the oracle's FP32 3-D FMM on one core.
@@ -455,6 +457,54 @@ def bench_precision(g, cfg: dict, speed, seeds, ref64, ref_fim, contrast=None) -
     return out
 
 
+def bench_reference_kernels(g, ctx, d_cost, d_time, seeds, steps: int) -> dict:
+    """The 4096^2 solve with the reference's own in-tile algorithms (the systolic kernel at
+    its default 8x8 blocks and at 32x8, the FIM kernel at 16x16), each with the reference's
+    default wave size (n = CUs) and in-tile relaxation until convergence: time per solve and
+    the tile kernel's roofline (algorithmic bytes per launch / average launch, HIP events)."""
+    import numpy as np
+
+    saved = {k: getattr(ctx.config, k) for k in ("tile_w", "tile_h", "kernel", "n_active", "inner_iter_cap")}
+    out = {}
+    ref = None
+    for name, kern, (tw, th) in (("systolic_8x8", g.KERNEL_SYSTOLIC, (8, 8)),
+                                 ("systolic_32x8", g.KERNEL_SYSTOLIC, (32, 8)),
+                                 ("fim_16x16", g.KERNEL_FIM, (16, 16))):
+        ctx.set_config(tile_w=tw, tile_h=th, kernel=kern, n_active=0, inner_iter_cap=0)
+
+        def step():
+            return ctx.solve_device(d_cost.ptr, g.FIELD_COST_F32, W, H, seeds, d_time.ptr)
+
+        step()
+        t0 = time.perf_counter()
+        sts = [step() for _ in range(steps)]
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        t = d_time.download()
+        ctx.set_config(profile_kernels=1)
+        psts = [step() for _ in range(steps)]
+        ctx.set_config(profile_kernels=0)
+        launches = sum(x["solve_launches"] for x in psts)
+        kern_us = sum(x["solve_kernel_us"] for x in psts)
+        visits = sum(x["tile_visits"] for x in psts)
+        bpl = visits * sts[0]["bytes_per_visit"] / max(launches, 1)
+        avg_us = kern_us / max(launches, 1)
+        achieved = bpl / (avg_us * 1e-6) / 1e9
+        if ref is None:
+            ref = t
+        out[name] = {"ms_per_solve": round(ms, 3), "mcells_per_s": round(W * H / (ms / 1e3) / 1e6, 1),
+                     "n_active": sts[0]["n_active"], "waves": sts[0]["waves"],
+                     "tile_visits": sts[0]["tile_visits"],
+                     "passes_per_visit": round(sts[0]["inner_iterations"] / max(1, sts[0]["tile_visits"]), 2),
+                     "max_abs_diff_vs_first": float(np.max(np.abs(t - ref))),
+                     "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": round(achieved, 2),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                  "bytes_per_visit": sts[0]["bytes_per_visit"], "avg_launch_us": round(avg_us, 3),
+                                  "launches_per_step": launches / steps}}
+        log(f"bench: reference kernel {name}: {ms:.2f} ms per solve, frac {achieved / HBM_PEAK_GBS:.4f}")
+    ctx.set_config(**saved)
+    return out
+
+
 def bench_volume(g, steps: int, cpu: bool) -> dict:
     """The 3-D (K = 3) solve on the 256^3 smooth volume: Mcells/s, the 3-D tile kernel's
     roofline, parity against the oracle's FP32 and FP64 3-D FMM, the oracle timed on one core."""
@@ -558,6 +608,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
     ap.add_argument("--no-volume", action="store_true", help="skip the 3-D volume line")
+    ap.add_argument("--no-reference-kernels", action="store_true",
+                    help="skip the 4096^2 solve with the systolic and FIM tile kernels")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this size first (rocprofv3 --pmc scaling)")
     args = ap.parse_args()
@@ -661,6 +713,9 @@ def main() -> int:
                             "mcells_per_s": round(W * H / (host_ms / 1e3) / 1e6, 1),
                             "note": "ghcm_solve with the field and times in host memory "
                                     "(128 MB over PCIe per solve)"}}
+
+    if not args.no_reference_kernels:
+        out["reference_kernels"] = bench_reference_kernels(g, ctx, d_cost, d_time, seeds, 3)
 
     plain_cfg = dict(tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
                      inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,
