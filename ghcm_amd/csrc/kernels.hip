@@ -409,7 +409,8 @@ __device__ __forceinline__ void write_edges(const SolveArgs& a, uint32_t g, int 
 template <int TW, int TH, int KIND, int MODE, bool HOST>
 __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uint32_t wave,
                              uint32_t nxt, uint32_t p, uint32_t job_slot,
-                             float* t_s, float* c_s, uint8_t* open_s, uint8_t* add_s) {
+                             float* t_s, float* c_s, uint8_t* open_s, uint8_t* add_s,
+                             uint16_t* list_s) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
     constexpr int P = TW * TH;
@@ -510,7 +511,10 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             }
         }
     } else {
-        // FIM body (data/eikonal_ghcm_fim.cl:290-372) with a byte open set in LDS.
+        // FIM body (data/eikonal_ghcm_fim.cl:290-372) with a byte open set in LDS, compacted
+        // into a list of open cells at the start of every pass (the reference compacts its
+        // open list too, data/eikonal_ghcm_fim.cl:51-134): both phases of a pass touch the open
+        // cells only, 64 per iteration, in cell-index order.
         for (int i = lane; i < P; i += 64) {
             int x = i % TW, y = i / TW;
             bool in_grid = x < sx && y < sy;
@@ -521,15 +525,27 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
             add_s[i] = 0;
         }
         __syncthreads();
+        const uint64_t below = (1ull << lane) - 1ull;
         for (;;) {
-            bool any_open = false;
+            // Compaction: each chunk of 64 cells appends its open ones (ballot, popcount).
+            uint32_t n_open = 0;
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                const int i = lane + 64 * k;
+                const bool o = i < P && open_s[i] != 0;
+                const uint64_t m = __ballot(o);
+                if (o) list_s[n_open + (uint32_t)__popcll(m & below)] = (uint16_t)i;
+                n_open += (uint32_t)__popcll(m);
+            }
+            if (n_open == 0u) break;
+            __syncthreads();
             bool conv[CPL];
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
-                int i = lane + 64 * k;
+                const uint32_t e = (uint32_t)(lane + 64 * k);
                 conv[k] = false;
-                if (i < P && open_s[i]) {
-                    any_open = true;
+                if (e < n_open) {
+                    const int i = list_s[e];
                     int q = (i / TW + 1) * PW + (i % TW) + 1;
                     float t0 = t_s[q];
                     float tn = cell_solve(q);
@@ -541,14 +557,14 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
                     conv[k] = !improved;          // converged iff diff > -eps
                 }
             }
-            if (__ballot(any_open) == 0ull) break;
             if (passes >= cap) { unfinished = true; break; }
             ++passes;
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
-                int i = lane + 64 * k;
-                if (i < P && conv[k]) {
+                const uint32_t e = (uint32_t)(lane + 64 * k);
+                if (e < n_open && conv[k]) {
+                    const int i = list_s[e];
                     const int x = i % TW, y = i / TW;
 #pragma unroll
                     for (int d = 0; d < 4; ++d) {
@@ -578,14 +594,19 @@ __device__ void process_tile(const SolveArgs& a, uint32_t g, uint32_t mbits, uin
                 }
             }
             __syncthreads();
+            // Converged open cells leave the set; cells added in this pass join it (they were
+            // not open, so the two writes never meet a cell twice).
 #pragma unroll
             for (int k = 0; k < CPL; ++k) {
-                int i = lane + 64 * k;
-                if (i < P) {
-                    uint8_t o = open_s[i];
-                    if (conv[k]) o = 0;
-                    if (add_s[i]) { o = 1; add_s[i] = 0; }
-                    open_s[i] = o;
+                const uint32_t e = (uint32_t)(lane + 64 * k);
+                if (e < n_open && conv[k]) open_s[list_s[e]] = 0;
+            }
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) {
+                const int i = lane + 64 * k;
+                if (i < P && add_s[i]) {
+                    open_s[i] = 1;
+                    add_s[i] = 0;
                 }
             }
             __syncthreads();
@@ -671,6 +692,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     __shared__ float c_s[PP];
     __shared__ uint8_t open_s[MODE == MODE_FIM ? P : 1];
     __shared__ uint8_t add_s[MODE == MODE_FIM ? P : 1];
+    __shared__ uint16_t list_s[MODE == MODE_FIM ? P : 1];
     const int lane = threadIdx.x;
 
     if constexpr (HOST) {
@@ -680,7 +702,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
         // A graph-replayed wave launches n work-groups; its unused job slots hold kNoTile.
         if (job.id == kNoTile) return;
         process_tile<TW, TH, KIND, MODE, true>(a, job.id, job.border, 0u, 0u, 0u, j,
-                                               t_s, c_s, open_s, add_s);
+                                               t_s, c_s, open_s, add_s, list_s);
     } else {
         __shared__ uint32_t s_prefix[kShards + 1];
         const GhcmCtrl* ctrl = a.ctrl;
@@ -711,7 +733,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
             if (lane == 0) mb = claim_tile(a, g, p);
             mb = __shfl(mb, 0);
             process_tile<TW, TH, KIND, MODE, false>(a, g, mb, wave, cur ^ 1u, p, j,
-                                                    t_s, c_s, open_s, add_s);
+                                                    t_s, c_s, open_s, add_s, list_s);
             __syncthreads();
         }
         if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
