@@ -48,6 +48,20 @@ def test_3d_parity_random_with_walls(shape, kernel, rng):
     _check(t, ref, cost, seeds, ctx.config.min_change)
 
 
+def test_3d_seeds_on_tile_edge_columns(rng):
+    """Seeds on the first and last x-column of 64x8x8 sweep tiles: the seed sets the tile's
+    edge column too, which its x-neighbours stage as their halo; a ragged last tile (x 192..199)
+    has no last column of its own to publish."""
+    cost = rng.uniform(0.2, 4.0, (12, 20, 200)).astype(np.float32)
+    seeds = [(63, 3, 2), (64, 17, 9), (127, 0, 0), (128, 19, 11), (192, 5, 1), (199, 10, 5)]
+    ref = oracle.fmm3d_f64(cost, seeds)
+    print("case 3d seeds on edge columns", flush=True)
+    with g.Context(0, n_active=64, **KERNELS["sweep"]) as ctx:
+        t, st = ctx.solve3d(cost, seeds)
+    assert st["converged"] == 1
+    _check(t, ref, cost, seeds, ctx.config.min_change)
+
+
 @pytest.mark.parametrize("kernel", list(KERNELS))
 def test_3d_slab_matches_2d(kernel, rng):
     cost2 = rng.uniform(0.1, 10.0, (64, 80)).astype(np.float32)
