@@ -74,25 +74,34 @@ __device__ __forceinline__ float load_cost_at(const char* p) {
 // update m + c applies (data/eikonal_common.cl:296-297); otherwise T = m + d/2 +
 // sqrt((sqrt2 c - d)(sqrt2 c + d))/2, the root of (T-a)^2 + (T-b)^2 = c^2 (:299-300) with
 // the discriminant factored and the large common part m added outside the root.
-// Branch-free: both candidates are formed and one is selected.  `!(c >= d)` also takes the
-// 1-D branch when d is NaN (a = b = +INF), which yields +INF.  The square root is the
-// hardware v_sqrt_f32 (1 ulp); its argument lies in [c^2, 2c^2] on the selected branch.
-// The large common part m is added by one final rounding of m + (d + root)/2, the
+// Branch-free, without a select: the two-axis form is evaluated at e = min(d, c), as
+// m + (e + max(root, e))/2.  For d <= c that is the form itself (root >= d there, exactly;
+// the max matters only within an ulp of d = c, where it gives m + d = max(a,b), the exact
+// value).  For d > c, e = c and the rounded root never exceeds c (sqrt2 rounds down to the
+// float constant), so the max yields c and the result is m + c rounded once: the 1-D update,
+// bit for bit.  The result therefore never falls as d crosses c (the 1-D value is not below
+// the two-axis values next to it), which the border checks rely on: they predict a
+// neighbour's update from a subset of the inputs its own solve sees, and a prediction an ulp
+// below what the solve then computes would re-activate the pair without end at eps = 0.
+// The discriminant lies in [c^2, 2c^2] on every lane, so the square root (the hardware
+// v_sqrt_f32, 1 ulp) never sees a negative argument.  e is a v_med3 against 0 (d and c are
+// non-negative); when d is NaN (a = b = +INF) the med3 yields the minimum of the other two
+// operands, 0, and the result is m + (a finite value) = +INF.
+// The large common part m is added by one final rounding of m + (e + root)/2, the
 // parenthesis formed at the scale of c: the result then never rounds below max(a,b) where
 // the exact root equals it (c = d), so a cell never drops below a neighbour it was computed
-// from.  (Rounding m + d/2 first and adding root/2 after breaks that: two cells can then
-// lower each other by an ulp at a time without end.)
+// from.  (Rounding m + e/2 first and adding root/2 after breaks that: two cells can then
+// lower each other by an ulp at a time without end.)  The final step is
+// fma(e + max(root, e), 1/2, m): halving is exact for normal numbers, so it equals m + (the
+// rounded half-sum) with one rounding.
 // Every update of a cell and every border prediction of a neighbour's update goes through
 // this one function.
-// (d + root)/2 is formed as fma(root, 1/2, d/2): halving is exact for normal numbers, so the
-// fma's one rounding of root/2 + d/2 equals half the rounded d + root, bit for bit, and d/2
-// is ready before the root, which takes one dependent operation off each step's chain.
 __device__ __forceinline__ float godunov_raw(float ax, float ay, float m, float c) {
     const float d = fabsf(ax - ay);                // NaN when both are +INF
+    const float e = __builtin_amdgcn_fmed3f(d, c, 0.0f);
     const float s = 1.41421356237309515f * c;
-    // (s - d)(s + d) >= 0 whenever c >= d; the other branch discards it.
-    const float two = __builtin_fmaf(__builtin_amdgcn_sqrtf((s - d) * (s + d)), 0.5f, 0.5f * d);
-    return m + ((c >= d) ? two : c);
+    const float root = __builtin_amdgcn_sqrtf((s - e) * (s + e));
+    return __builtin_fmaf(e + __builtin_fmaxf(root, e), 0.5f, m);
 }
 
 // The update min'ed with the old time t (:305-308).
@@ -756,10 +765,13 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 constexpr int kSweepThreads = 256;
 constexpr uint32_t kSweepHardRounds = 64;
 
-__device__ __forceinline__ float dpp_from_lower_lane(float old, float v) {
-    // wave_shr:1 -- lane i receives lane i-1's v; lane 0 keeps `old`.
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138,
-                                                      0xF, 0xF, false));
+// min(u, lane L-1's v) on time bits, with lane 0 (no source lane) taking u.  The DPP move's
+// `old` is the minimum's identity (all ones) and its bound control is off, so the compiler
+// folds the move into the minimum as one v_min_u32 with a wave_shr:1 DPP operand, whose
+// lane 0 keeps u.
+__device__ __forceinline__ float umin_from_lower_lane(float u, float v) {
+    const uint32_t w = (uint32_t)__builtin_amdgcn_update_dpp(-1, __float_as_int(v), 0x138, 0xF, 0xF, false);
+    return __uint_as_float(min(__float_as_uint(u), w));
 }
 
 // LDS words of a sweep tile: time box, a sink word, cost box.
@@ -783,10 +795,10 @@ constexpr int sweep_lds_words() { return 2 * (TW + 2) * (TH + 2) + 1; }
 // `prev` (this lane's previous cell, the upwind row) and a wave_shr:1 DPP move of lane L-1's
 // `prev` (the upwind column); each is merged with this lane's own LDS read of that cell, so
 // improvements made by the other three sweeps are used as soon as they land.  Lane 0 has no
-// source lane: `lreg` keeps +INF there and its upwind column is the halo read from LDS.
+// source lane: its upwind column is the halo read from LDS alone.
 // Results are merged into the tile with an LDS atomic minimum (no return value waited for).
 template <int TW, int TH, int DX, bool DOWN>
-__device__ __forceinline__ void sweep_round(char* const shb, const int L, float& lreg) {
+__device__ __forceinline__ void sweep_round(char* const shb, const int L) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
     constexpr int CB = PW * PH + 1;
@@ -842,8 +854,7 @@ __device__ __forceinline__ void sweep_round(char* const shb, const int L, float&
     float prev = __builtin_inff();
     auto step = [&](const Nb& n) {
         // Lane L-1's fresh value of the upwind column cell.
-        lreg = dpp_from_lower_lane(lreg, prev);
-        const float ax = umin(n.uc, lreg);
+        const float ax = umin_from_lower_lane(n.uc, prev);
         const float ay = umin(n.ur, n.first ? __builtin_inff() : prev);
         const float t = godunov_raw(ax, ay, umin(ax, ay), n.cc);
         if (TW == 64 || lane_on)
@@ -913,16 +924,13 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     float snap[CPT];
 #pragma unroll
     for (int k = 0; k < CPT; ++k) snap[k] = told[k];
-    // Destination of the DPP move, carried from step to step and round to round: lane 0 has no
-    // source lane, is never written and keeps +INF.
-    float lreg = __builtin_inff();
     for (;;) {
         // Wave w runs sweep order w (the branch is wave-uniform).
         switch (w) {
-            case 0: sweep_round<TW, TH, 1, true>(shb, L, lreg); break;
-            case 1: sweep_round<TW, TH, -1, true>(shb, L, lreg); break;
-            case 2: sweep_round<TW, TH, 1, false>(shb, L, lreg); break;
-            default: sweep_round<TW, TH, -1, false>(shb, L, lreg); break;
+            case 0: sweep_round<TW, TH, 1, true>(shb, L); break;
+            case 1: sweep_round<TW, TH, -1, true>(shb, L); break;
+            case 2: sweep_round<TW, TH, 1, false>(shb, L); break;
+            default: sweep_round<TW, TH, -1, false>(shb, L); break;
         }
         ++rounds;
         __syncthreads();
@@ -1949,8 +1957,9 @@ __device__ __forceinline__ float godunov3_sel(float a, float b, float c, float f
     // inputs are sorted), shared with the three-axis form.
     const float u = m2 - m1;
     const float s2 = 1.41421356237309515f * f;
-    const float two = __builtin_fmaf(__builtin_amdgcn_sqrtf((s2 - u) * (s2 + u)), 0.5f, 0.5f * u);
-    const float t2 = m1 + ((f >= u) ? two : f);
+    const float e = __builtin_amdgcn_fmed3f(u, f, 0.0f);
+    const float r2 = __builtin_amdgcn_sqrtf((s2 - e) * (s2 + e));
+    const float t2 = __builtin_fmaf(e + __builtin_fmaxf(r2, e), 0.5f, m1);
     const float v = m3 - m1, w = u - v;
     const float disc = fmaxf(3.0f * f * f - (u * u + v * v + w * w), 0.0f);
     const float t3 = m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
