@@ -1919,54 +1919,43 @@ __global__ void k_rgb_luminosity(const float* __restrict__ rgb, float* __restric
 
 // Godunov upwind update in three dimensions for |grad T| = f, h = 1 (the K-axis update of
 // src/eikonal.h:73-118 and data/eikonal_common.cl:221-285 at K = 3), on the three axis
-// minima a, b, c in any order.  Sorted m1 <= m2 <= m3, the update takes the first of
+// minima a, b, c in any order.  Sorted m1 <= m2 <= m3, the update is the first of
 //   m1 + f                                    if it does not exceed m2,
-//   the two-axis root (godunov_raw)           if it does not exceed m3,
+//   the two-axis root                         if it does not exceed m3,
 //   m1 + (u + v + sqrt(3 f^2 - u^2 - v^2 - (u - v)^2)) / 3, u = m2 - m1, v = m3 - m1,
 // the larger root of (T-m1)^2 + (T-m2)^2 + (T-m3)^2 = f^2 with the common part m1 taken out
-// of the discriminant (the reference forms it from the unshifted values).  As in two
-// dimensions, m1 is added by one final rounding, so a result never rounds below the
-// largest minimum it used.  +INF minima fall through to the lower-axis forms; f = +INF
-// yields +INF.
-__device__ __forceinline__ float godunov3_raw(float a, float b, float c, float f) {
-    const float lo = fminf(a, b), hi = fmaxf(a, b);
-    const float m1 = fminf(lo, c);
-    const float x = fmaxf(lo, c);
-    const float m2 = fminf(x, hi), m3 = fmaxf(x, hi);
-    const float t1 = m1 + f;
-    if (!(t1 > m2)) return t1;
-    const float t2 = godunov_raw(m1, m2, m1, f);
-    if (!(t2 > m3)) return t2;
-    const float u = m2 - m1, v = m3 - m1, w = u - v;
-    const float disc = fmaxf(3.0f * f * f - (u * u + v * v + w * w), 0.0f);
-    return m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
-}
-
-// godunov3_raw with both candidates formed and the result selected, for the sweep step, where
-// the lanes of a wave take different branches and every branch would run anyway.  The same
-// operations in the same order: the result equals godunov3_raw's bit for bit.  The inputs are
-// non-negative or +INF, so the sort uses integer minimum and maximum.
-__device__ __forceinline__ float godunov3_sel(float a, float b, float c, float f) {
+// of the discriminant (the reference forms it from the unshifted values).
+// Evaluated without a compare or select, by clamping each added axis to where the form
+// below it stops being valid (the forms meet there):
+//   t2 = the two-axis form at e = min(u, f) (as godunov_raw: exactly m1 + f for u >= f),
+//   m3c = min(m3, t2): for m3 >= t2 the three-axis root at v = t2 - m1 is t2 itself,
+//   t3 = the three-axis form at (e, m3c - m1),
+//   T = med3(t3, m3c, t2), i.e. t3 clamped to [m3c, t2].
+// For m3 >= t2 that is t2 exactly; for m3 < t2 it is t3, never below m3 (the largest minimum
+// it used) and never above t2.  A float32 emulation over 3 M inputs spread over the three
+// branches and their boundaries gives the branch form's result bit for bit except 32
+// three-axis cases (1-2 ulp, where t3 had rounded above t2), and the same count of one-ulp
+// non-monotone steps.  As in two dimensions, m1 is added by one final rounding.  +INF minima
+// fall through to the lower-axis forms (an INF - INF difference makes a NaN that the clamps
+// and the final med3, which returns the minimum of its other operands for a NaN operand, drop);
+// f = +INF yields +INF.  Every cell update and every face prediction of both 3-D kernels goes
+// through this one function.
+__device__ __forceinline__ float godunov3(float a, float b, float c, float f) {
     // Sorted by three-operand minimum, median and maximum (v_min3/v_med3/v_max3: the values
     // are non-negative or +INF, never NaN, so the float forms order them exactly).
     const float m1 = fminf(fminf(a, b), c);
     const float m3 = fmaxf(fmaxf(a, b), c);
     const float m2 = __builtin_amdgcn_fmed3f(a, b, c);
-    const float t1 = m1 + f;
-    // godunov_raw(m1, m2, m1, f) with its |m1 - m2| formed as m2 - m1 (the same value: the
-    // inputs are sorted), shared with the three-axis form.
     const float u = m2 - m1;
     const float s2 = 1.41421356237309515f * f;
     const float e = __builtin_amdgcn_fmed3f(u, f, 0.0f);
     const float r2 = __builtin_amdgcn_sqrtf((s2 - e) * (s2 + e));
     const float t2 = __builtin_fmaf(e + __builtin_fmaxf(r2, e), 0.5f, m1);
-    const float v = m3 - m1, w = u - v;
-    const float disc = fmaxf(3.0f * f * f - (u * u + v * v + w * w), 0.0f);
-    const float t3 = m1 + (u + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
-    // Both candidates exist here on every lane, so the selection is two v_cndmask, not a
-    // branch around their computation (a branch adds scalar instructions to every step).
-    __asm__ volatile("" ::"v"(t2), "v"(t3));
-    return !(t1 > m2) ? t1 : !(t2 > m3) ? t2 : t3;
+    const float m3c = __builtin_fminf(m3, t2);
+    const float v = m3c - m1, w = e - v;
+    const float disc = fmaxf(3.0f * f * f - (e * e + v * v + w * w), 0.0f);
+    const float t3 = m1 + (e + v + __builtin_amdgcn_sqrtf(disc)) * (1.0f / 3.0f);
+    return __builtin_amdgcn_fmed3f(t3, m3c, t2);
 }
 
 constexpr int kT3 = 8;                 // 3-D tile edge
@@ -2062,7 +2051,7 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
                 const float ax = fminf(t_s[c - 1], t_s[c + 1]);
                 const float ay = fminf(t_s[c - PB], t_s[c + PB]);
                 const float az = fminf(t_s[c - PB2], t_s[c + PB2]);
-                const float tn = godunov3_raw(ax, ay, az, cost[k]);
+                const float tn = godunov3(ax, ay, az, cost[k]);
                 if (tn < t0) {
                     t_s[c] = tn;
                     tcur[k] = tn;
@@ -2108,7 +2097,7 @@ __device__ void solve3d_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint
         if (s == 4 || s == 5) az = t_s[c + (s == 4 ? PB2 : -PB2)];
         else az = fminf(t_s[c - PB2], t_s[c + PB2]);
         const float t0 = t_s[c];
-        const float tn = godunov3_raw(ax, ay, az, c_s[c]);
+        const float tn = godunov3(ax, ay, az, c_s[c]);
         if (tn < t0 && t0 - tn >= eps) {
             atomicOr(&s_u32[s], 1u);
             atomicMin(reinterpret_cast<uint32_t*>(s_f32) + s, __float_as_uint(tn));
@@ -2237,11 +2226,15 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
     constexpr int kUpY = kCell - SY * PX;
     constexpr int kUpZ = kCell - SZ * PL;
     constexpr int kCost = kS3Box + kCell;
-    // Rows whose upwind y (z) neighbour is in the halo: y = 0 (7) for SY > 0 (< 0), z = 0 (7).
-    constexpr uint64_t kYHalo = SY > 0 ? 0x0101010101010101ull : 0x8080808080808080ull;
+    // Rows whose upwind z neighbour is in the halo: z = 0 (7) for SZ > 0 (< 0).
     constexpr uint64_t kZHalo = SZ > 0 ? 0x00000000000000FFull : 0xFF00000000000000ull;
     // DPP row shift by one lane towards higher lanes (SY > 0: lane L reads lane L-1) or lower.
+    // The rows whose upwind y neighbour is in the halo (y = 0 for SY > 0, 7 for SY < 0) either
+    // have no source lane in their row of 16 or take the row y = 7 (0) of the neighbouring z:
+    // those rows (kYLast) hand on +INF instead of their result, so every halo row's y value is
+    // its LDS read alone.
     constexpr int kDppY = SY > 0 ? 0x111 : 0x101;
+    constexpr uint64_t kYLast = SY > 0 ? 0x8080808080808080ull : 0x0101010101010101ull;
     const int y = L & 7, z = L >> 3;
     const int rank = (SY > 0 ? y : kS3Y - 1 - y) + (SZ > 0 ? z : kS3Z - 1 - z);
     const int zsrc = 4 * ((L - 8 * SZ) & 63);           // ds_bpermute byte address of the z-upwind lane
@@ -2263,37 +2256,41 @@ __device__ __forceinline__ void sweep3_round(char* const shb, const int L) {
         return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(shb + b + 4 * word),
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
     };
-    // Step s: lanes with rank == s mod 64 are at x-position 0 (their walk restarts at rs).
+    // Step s: lanes with rank == s mod 64 are at x-position 0 (their walk restarts at rs).  The
+    // test for the next step is made one load ahead, so the select does not wait on the
+    // compare's lane mask.
+    bool first_next = rank == 0;
     auto load = [&](Nb& o, int st) {
-        o.first = rank == (st & 63);
+        o.first = first_next;
         bb = o.first ? rs : bb + 4 * DX;
         o.b = bb;
         o.ux = at(bb, kUpX);
         o.uy = at(bb, kUpY);
         o.uz = at(bb, kUpZ);
         o.cc = at(bb, kCost);
+        first_next = rank == ((st + 1) & 63);
     };
     auto umin = [](float u, float v) {
         return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v)));
     };
     const float inf = __builtin_inff();
-    // The previous step's result of this lane (upwind x), of the y-upwind lane (DPP row shift)
-    // and of the z-upwind lane (ds_bpermute).  Both moves are issued as soon as the result
-    // exists, ahead of the next step's LDS reads, so waiting for the permute does not wait for
-    // them.  The DPP destination is carried from step to step: lanes without a source lane keep
-    // their old value, and every such lane is one the y-halo mask replaces.
-    float prev = inf, yv = inf, zv = inf;
+    // The previous step's result of this lane (upwind x), of the y-upwind lane (a DPP row shift
+    // folded into the minimum: its `old` is the minimum's identity and its bound control is
+    // off, so a lane without a source lane keeps its LDS read) and of the z-upwind lane
+    // (ds_bpermute, issued as soon as the result exists, ahead of the next step's LDS reads).
+    float prev = inf, ysend = inf, zv = inf;
     auto step = [&](const Nb& n) {
         const float ax = umin(n.ux, n.first ? inf : prev);
-        const float ay = umin(n.uy, __builtin_amdgcn_inverse_ballot_w64(kYHalo) ? inf : yv);
+        const float ay = __uint_as_float(
+            min(__float_as_uint(n.uy),
+                (uint32_t)__builtin_amdgcn_update_dpp(-1, __float_as_int(ysend), kDppY, 0xF, 0xF, false)));
         const float az = umin(n.uz, __builtin_amdgcn_inverse_ballot_w64(kZHalo) ? inf : zv);
-        const float t = godunov3_sel(ax, ay, az, n.cc);
+        const float t = godunov3(ax, ay, az, n.cc);
         __hip_atomic_fetch_min(reinterpret_cast<uint32_t*>(shb + n.b + 4 * kCell), __float_as_uint(t),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         prev = t;
         zv = __int_as_float(__builtin_amdgcn_ds_bpermute(zsrc, __float_as_int(t)));
-        yv = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(yv), __float_as_int(t), kDppY, 0xF, 0xF,
-                                                        false));
+        ysend = __builtin_amdgcn_inverse_ballot_w64(kYLast) ? inf : t;
     };
     // Loads of step s+1 are issued before step s is computed (two register sets): their cells
     // are upper bounds whatever the interleaving, and the values step s produces reach step
@@ -2517,7 +2514,7 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
     // One code path for every face: the block's face gives (wave-uniform) the halo cell's box
     // word as base + u * su + v * sv for in-face coordinates (u, v) = (f & mu, f >> shv), the
     // word offset of its one staged neighbour inside the tile, and the offsets of its two
-    // lateral pairs; godunov3_sel sorts its three axis values, so their order is free.
+    // lateral pairs; godunov3 sorts its three axis values, so their order is free.
     const int ntx = (int)in.tiles_x, nty = (int)in.tiles_y, ntz = (int)in.tiles_z;
     auto umin = [](float u, float v) { return __uint_as_float(min(__float_as_uint(u), __float_as_uint(v))); };
     for (int bi = w; bi < 34; bi += 8) {
@@ -2547,7 +2544,7 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
             const float a_1 = umin(t_s[c - l1], t_s[c + l1]);
             const float a_2 = umin(t_s[c - l2], t_s[c + l2]);
             const float t0 = t_s[c];
-            const float tn = godunov3_sel(a_in, a_1, a_2, c_s[c]);
+            const float tn = godunov3(a_in, a_1, a_2, c_s[c]);
             if (tn < t0 && t0 - tn >= eps) key = tn;
         }
         // Most blocks predict no improvement: only the others reduce their key and merge it.
