@@ -934,6 +934,9 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         }
         ++rounds;
         __syncthreads();
+        // With one round per visit its change test is made in the commit pass below, which
+        // reads the same cells (the round began at the staged times, `told`).
+        if (cap == 1u) break;
         bool chg = false;
 #pragma unroll
         for (int k = 0; k < CPT; ++k) {
@@ -954,11 +957,16 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         if (!more) break;
         if (rounds >= cap) { unfinished = true; break; }
     }
+    // One round per visit: whether it changed the tile (then the tile re-queues itself) is
+    // tested against `told` in the commit pass (cells outside the grid are +INF and never
+    // change).
+    const bool one_round = cap == 1u;
     // Commit improved cells first (plain stores: one writer per tile per wave), so the stores
     // drain while the border check and the activations run; the tile min is reduced after.
     char* const cb = reinterpret_cast<char*>(in.time) + ((int64_t)oy * W + ox) * 4;
     float tmin = __builtin_inff();
     uint32_t improved = 0;
+    bool last_chg = false;
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
         const int i = tid + kSweepThreads * k;
@@ -970,9 +978,14 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
                 if (tn < told[k]) {
                     *gmem(reinterpret_cast<float*>(cb + ((uint32_t)cy * (uint32_t)W + (uint32_t)cx) * 4u)) = tn;
                     ++improved;
+                    if (told[k] - tn >= eps) last_chg = true;
                 }
             }
         }
+    }
+    {
+        const bool wave_chg = __ballot(one_round && last_chg) != 0ull;
+        if (L == 0) s_u32[12 + w] = wave_chg ? 1u : 0u;
     }
     write_edges<TW, TH>(a, g, sx, sy, w, L, t_s);
     // Border check, wave w takes side w: would the neighbour's adjacent cell improve by
@@ -1026,6 +1039,7 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     }
     __syncthreads();
     const float tm = fminf(fminf(s_f32[4], s_f32[5]), fminf(s_f32[6], s_f32[7]));
+    if (one_round) unfinished = (s_u32[12] | s_u32[13] | s_u32[14] | s_u32[15]) != 0u;
     if constexpr (!HOST) {
         if (tid < 4 && s_u32[4 + tid]) {
             const int sd = tid;
