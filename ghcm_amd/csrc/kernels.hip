@@ -318,7 +318,7 @@ __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, i
 template <int TW, int TH, int KIND>
 __device__ __forceinline__ void stage_sweep_tile(const SolveArgs& a, const GhcmInst& in, uint32_t g,
                                                  int tx, int ox, int oy, int tid, float* t_s,
-                                                 float* c_s) {
+                                                 float* c_s, float* told) {
     constexpr int NT = 256;
     constexpr int PW = TW + 2, PH = TH + 2, P = TW * TH;
     constexpr int NI = (P + NT - 1) / NT;        // interior cells per thread
@@ -383,6 +383,9 @@ __device__ __forceinline__ void stage_sweep_tile(const SolveArgs& a, const GhcmI
             t_s[q] = tv[k];
             c_s[q] = cv[k];
         }
+        // The staged time of interior cell tid + NT k (+INF outside the grid or the tile):
+        // the caller's snapshot, which it would otherwise read back from LDS.
+        told[k] = tv[k];
     }
 #pragma unroll
     for (int k = 0; k < NH; ++k) {
@@ -904,14 +907,11 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const int W = (int)in.w, H = (int)in.h;
     const int sx = tile_extent(W, ox, TW), sy = tile_extent(H, oy, TH);
 
-    stage_sweep_tile<TW, TH, KIND>(a, in, g, tx, ox, oy, tid, t_s, c_s);
-    __syncthreads();
+    // told[k]: the staged time of cell tid + 256 k, the snapshot the commit compares with.
+    static_assert(kSweepThreads == 256, "staging and the commit map cells to threads alike");
     float told[CPT];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        int i = tid + kSweepThreads * k;
-        told[k] = (i < P) ? t_s[(i / TW + 1) * PW + (i % TW) + 1] : __builtin_inff();
-    }
+    stage_sweep_tile<TW, TH, KIND>(a, in, g, tx, ox, oy, tid, t_s, c_s, told);
+    __syncthreads();
 
     const float eps = a.eps;
     const uint32_t cap = (a.iter_cap == 0u || a.iter_cap > kSweepHardRounds) ? kSweepHardRounds : a.iter_cap;
@@ -2477,6 +2477,9 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
         }
         ++rounds;
         __syncthreads();
+        // With one round per visit its change test is made in the commit pass below, which
+        // reads the same cells (the round began at the staged times, `told`).
+        if (cap == 1u) break;
         bool chg = false;
 #pragma unroll
         for (int k = 0; k < kS3Z; ++k) {
@@ -2497,9 +2500,14 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
         if (rounds >= cap) { unfinished = true; break; }
     }
     // Commit improved cells (one writer per tile per wave): wave w writes row y = w of every
-    // z-plane, 64 contiguous cells each.
+    // z-plane, 64 contiguous cells each.  With one round per visit, whether it changed the
+    // tile (then the tile re-queues itself) is tested here against `told` (cells outside the
+    // grid are +INF and never change); each wave's answer goes to s_u32[w], which that path
+    // leaves unused, and is read after the barrier before the activations.
+    const bool one_round = cap == 1u;
     float tmin = __builtin_inff();
     uint32_t improved = 0;
+    bool last_chg = false;
 #pragma unroll
     for (int k = 0; k < kS3Z; ++k) {
         if (L < sx && w < sy && k < sz) {
@@ -2509,8 +2517,14 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
                 const size_t idx = ((size_t)(oz + k) * (size_t)H + (size_t)(oy + w)) * (size_t)W + (size_t)(ox + L);
                 gmem(in.time)[idx] = tn;
                 ++improved;
+                if (told[k] - tn >= eps) last_chg = true;
             }
         }
+    }
+    if (one_round) {
+        // Only on this path: after a round check other waves may still be reading s_u32[0..7].
+        const bool wave_chg = __ballot(last_chg) != 0ull;
+        if (L == 0) s_u32[w] = wave_chg ? 1u : 0u;
     }
     // Edge columns from the final box: wave 0 the first column, wave 1 the last one when the
     // tile is full width (only then has it a right neighbour that reads it); lane L is row
@@ -2578,6 +2592,12 @@ __device__ void sweep3_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint3
     float tm = s_f32[0];
 #pragma unroll
     for (int i = 1; i < 8; ++i) tm = fminf(tm, s_f32[i]);
+    if (one_round) {
+        uint32_t any = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) any |= s_u32[i];
+        unfinished = any != 0u;
+    }
     // Every activation after one barrier: the faces' neighbours (lanes 0-5 of wave 0) and,
     // when the round cap stopped the tile, the tile itself (wave 1).
     if (tid < 6) {

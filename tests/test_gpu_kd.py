@@ -1,5 +1,7 @@
 """Three-dimensional solves (the K = 3 generalisation of src/eikonal.h:73-118) on the GPU,
 checked against the oracle's 3-D FMM in FP64 -- needs an MI355X."""
+import functools
+
 import numpy as np
 import pytest
 
@@ -13,10 +15,15 @@ TOL = 1e-4      # max |dT| <= TOL * max T, as for the plain 2-D form
 
 # The two 3-D tile solvers: 8x8x8 tiles relaxed by in-tile passes (every kernel but the
 # sweep), and 64x8x8 tiles relaxed by the eight octant sweeps (the context's 2-D tile shape
-# must still be one the sweep kernel is built for).
+# must still be one the sweep kernel is built for), the latter relaxed until converged in
+# each visit (cap 0), one round per visit (cap 1: the change test made in the commit pass,
+# the configuration the volume is measured in) and two (cap 2: a round check, then a capped
+# last round).
 KERNELS = {"passes": dict(kernel=g.KERNEL_SYSTOLIC),
-           "sweep": dict(kernel=g.KERNEL_SWEEP, tile_w=64, tile_h=64)}
-SHAPES = {"passes": (8, 8, 8), "sweep": (64, 8, 8)}
+           "sweep": dict(kernel=g.KERNEL_SWEEP, tile_w=64, tile_h=64),
+           "sweep_cap1": dict(kernel=g.KERNEL_SWEEP, tile_w=64, tile_h=64, inner_iter_cap=1),
+           "sweep_cap2": dict(kernel=g.KERNEL_SWEEP, tile_w=64, tile_h=64, inner_iter_cap=2)}
+SHAPES = {"passes": (8, 8, 8), "sweep": (64, 8, 8), "sweep_cap1": (64, 8, 8), "sweep_cap2": (64, 8, 8)}
 
 
 def _check(t, ref, cost, seeds, eps):
@@ -145,16 +152,22 @@ def test_3d_volume_128_vs_fp64(kernel):
     _check(t, ref, cost, seeds, ctx.config.min_change)
 
 
-@pytest.mark.parametrize("kernel,n_active", [("passes", 4096), ("sweep", 256)])
+@functools.lru_cache(maxsize=1)
+def _volume_256():
+    from ghcm_amd import workloads
+    speed, seeds = workloads.volume_256()
+    cost = (np.float32(1.0) / speed).astype(np.float32)
+    return cost, seeds, oracle.fmm3d_f64(cost.astype(np.float64), seeds)
+
+
+@pytest.mark.parametrize("kernel,n_active", [("passes", 4096), ("sweep", 256), ("sweep_cap1", 2048)])
 def test_3d_long_lists_carried_by_the_tile_launch(kernel, n_active):
     """The 256^3 bench volume: with 8x8x8 tiles at n = 4096 its boundaries select from lists
     of 4-6 thousand tiles, with 64x8x8 tiles at n = 256 from lists of several hundred, and
     hand the unselected ones to the tile launch, which carries them to the next wave;
-    nothing may be lost or queued twice."""
-    from ghcm_amd import workloads
-    speed, seeds = workloads.volume_256()
-    cost = (np.float32(1.0) / speed).astype(np.float32)
-    ref = oracle.fmm3d_f64(cost.astype(np.float64), seeds)
+    nothing may be lost or queued twice.  The last case is the measured volume configuration
+    (one round per visit, n = 2048)."""
+    cost, seeds, ref = _volume_256()
     print(f"case 3d long lists 256^3 {kernel} n={n_active}", flush=True)
     with g.Context(0, n_active=n_active, **KERNELS[kernel]) as ctx:
         t, st = ctx.solve3d(cost, seeds)

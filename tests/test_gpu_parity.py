@@ -64,7 +64,7 @@ def test_rings256_every_config(rings256, kernel, tile, sched):
                                                (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
                                                (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_HOST_PQ, 0),
                                                (g.SCHED_HOST_PQ, 2)])
-@pytest.mark.parametrize("cap", [0, 1])
+@pytest.mark.parametrize("cap", [0, 1, 2])
 def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     cost, seeds, ref32, ref64 = rings256
     print(f"case rings256 sweep tile={tile} sched={sched} launch_mode={launch_mode} cap={cap}", flush=True)
