@@ -309,6 +309,9 @@ __device__ __forceinline__ void stage_tile(const GhcmInst& in, int ox, int oy, i
     }
 }
 
+// Threads of a sweep work-group: four wave64s, one per sweep order.
+constexpr int kSweepThreads = 256;
+
 // Stage a sweep tile: the same PW x PH box as stage_tile (tile plus halo ring, out-of-grid
 // cells +INF), read so that no cache line is fetched for a few of its bytes.  The interior is
 // TH aligned rows of TW cells of the time and cost fields; the top and bottom halo rows are
@@ -319,7 +322,7 @@ template <int TW, int TH, int KIND>
 __device__ __forceinline__ void stage_sweep_tile(const SolveArgs& a, const GhcmInst& in, uint32_t g,
                                                  int tx, int ox, int oy, int tid, float* t_s,
                                                  float* c_s, float* told) {
-    constexpr int NT = 256;
+    constexpr int NT = kSweepThreads;
     constexpr int PW = TW + 2, PH = TH + 2, P = TW * TH;
     constexpr int NI = (P + NT - 1) / NT;        // interior cells per thread
     constexpr int NHX = 2 * PW + 2 * TH;         // halo cells: two rows with corners, two columns
@@ -757,15 +760,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
 // the four Gauss-Seidel orders of fast sweeping (Zhao 2005, the method
 // data/eikonal_common.cl:227 cites), one order per wave, concurrently on the same LDS tile.
 // Lane L owns column x (x = L for sweeps towards +x, TW-1-L towards -x) and handles row
-// index s-L at step s (a skewed anti-diagonal front), so the upwind column neighbour of
-// the current row was produced by lane L-1 one step earlier: it arrives through a
-// wave_shr:1 DPP move, and the upwind row neighbour is the lane's own previous result.
-// The other neighbours are read from LDS (any value there is an upper bound, possibly
-// already lowered by another wave), and improvements are written back with plain LDS
-// stores.  A round is the four sweeps; rounds repeat until no cell of the tile improves by
-// >= eps or the round cap is reached (the tile then re-queues itself).
+// index (s-L) mod TH at step s (a wrapped, skewed anti-diagonal front), so the upwind column
+// neighbour of the current row was produced by lane L-1 one step earlier: it arrives as a
+// wave_shr:1 DPP operand of the minimum that merges it with the LDS read, and the upwind row
+// neighbour is the lane's own previous result.  Every neighbour is also read from LDS (any
+// value there is an upper bound, possibly already lowered by another wave), and results are
+// merged into the tile with an LDS atomic minimum.  A round is the four sweeps; rounds repeat
+// until no cell of the tile improves by >= eps or the round cap is reached (the tile then
+// re-queues itself).
 // ---------------------------------------------------------------------------------------
-constexpr int kSweepThreads = 256;
 constexpr uint32_t kSweepHardRounds = 64;
 
 // min(u, lane L-1's v) on time bits, with lane 0 (no source lane) taking u.  The DPP move's
@@ -907,8 +910,8 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
     const int W = (int)in.w, H = (int)in.h;
     const int sx = tile_extent(W, ox, TW), sy = tile_extent(H, oy, TH);
 
-    // told[k]: the staged time of cell tid + 256 k, the snapshot the commit compares with.
-    static_assert(kSweepThreads == 256, "staging and the commit map cells to threads alike");
+    // told[k]: the staged time of cell tid + kSweepThreads k (staging maps cells to threads as
+    // the commit does), the snapshot the commit compares with.
     float told[CPT];
     stage_sweep_tile<TW, TH, KIND>(a, in, g, tx, ox, oy, tid, t_s, c_s, told);
     __syncthreads();
