@@ -557,8 +557,10 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     } else {
         if ((rc = ctx->inst.reserve(sizeof(GhcmInst) * n_inst))) return rc;
         if ((rc = ctx->qwave.reserve(4ull * T))) return rc;
-        if ((rc = ctx->key.reserve(8ull * T))) return rc;
-        if ((rc = ctx->mask.reserve(8ull * T))) return rc;
+        // activation records: two parities, three when the boundary-free launch makes the
+        // top-n cut (launch_mode 5 rotates them with the lists)
+        if ((rc = ctx->key.reserve(12ull * T))) return rc;
+        if ((rc = ctx->mask.reserve(12ull * T))) return rc;
         if ((rc = ctx->tmin.reserve(4ull * T))) return rc;
         // three lists: boundary-free waves rotate them (the other modes use two)
         if ((rc = ctx->list.reserve(12ull * kShards * shard_cap))) return rc;
@@ -674,9 +676,12 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     const bool fused = !three_d && !host_sched && cfg.launch_mode == 3 && fk.launch != nullptr;
     const PersistentKernel pk = get_persistent_kernel((int)tw, (int)th, kind, cfg.kernel);
     const bool persistent = !three_d && !host_sched && cfg.launch_mode == 1 && pk.launch != nullptr;
-    // launch_mode 4: boundary-free waves (sweep kernel): one tile launch per wave.
-    const BfKernel bk = get_bf_kernel((int)tw, (int)th, kind, cfg.kernel, wide);
-    const bool bfree = !three_d && !host_sched && cfg.launch_mode == 4 && bk.launch != nullptr;
+    // launch_mode 4: boundary-free waves (sweep kernel): one tile launch per wave; launch_mode 5:
+    // the same with the top-n cut made by the tile launch (64 x 64 and 64 x 32 tiles; other
+    // shapes run the per-wave launches of launch_mode 0).
+    const bool bsel = cfg.launch_mode == 5;
+    const BfKernel bk = get_bf_kernel((int)tw, (int)th, kind, cfg.kernel, wide, bsel);
+    const bool bfree = !three_d && !host_sched && (cfg.launch_mode == 4 || bsel) && bk.launch != nullptr;
     if (bfree) {
         // ---- boundary-free device wave loop ----
         // Each tile launch reads its own list's counts; its first work-group publishes

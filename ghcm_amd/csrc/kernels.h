@@ -30,13 +30,15 @@ void tile3d_shape(bool sweep, uint32_t& tw, uint32_t& th, uint32_t& td);
 SolveKernel get_fused_kernel(int tile_w, int tile_h, int kind, int mode);
 
 // Boundary-free waves (sweep kernel): one tile launch per wave, the wave index as argument; no
-// wave-boundary kernel and no top-n cut (every queued tile is solved in its wave).
+// wave-boundary kernel.  Without `select` every queued tile is solved in its wave (no top-n
+// cut); with it the launch makes the top-n cut itself (three activation records; 64 x 64 and
+// 64 x 32 tiles), and `grid` counts the solving work-groups (the launch adds the carrying ones).
 struct BfKernel {
     hipError_t (*launch)(const SolveArgs&, uint32_t wave, uint32_t grid, hipStream_t,
                          hipEvent_t ev0, hipEvent_t ev1);
     int (*blocks_per_cu)();
 };
-BfKernel get_bf_kernel(int tile_w, int tile_h, int kind, int mode, bool wide);
+BfKernel get_bf_kernel(int tile_w, int tile_h, int kind, int mode, bool wide, bool select = false);
 
 // Persistent device wave loop (cooperative launch); null when not built for the combination.
 struct PersistentKernel {

@@ -179,14 +179,20 @@ __device__ __forceinline__ void list_append(const SolveArgs& a, uint32_t which, 
 // Queue tile nb for the next wave unless it is already queued, recording why (border
 // side bit) and the smallest candidate time that activated it.  `nxt` is the list that
 // collects the next wave's tiles.
-__device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32_t wave,
-                                         uint32_t nxt, uint32_t p, uint32_t mask_bits,
-                                         float key) {
-    const uint32_t np = p ^ 1u;
+__device__ __forceinline__ void activate_np(const SolveArgs& a, uint32_t nb, uint32_t wave,
+                                            uint32_t nxt, uint32_t np, uint32_t mask_bits,
+                                            float key) {
     atomicMin(&a.key[(size_t)np * a.tiles + nb], __float_as_uint(key));
     atomicOr(&a.mask[(size_t)np * a.tiles + nb], mask_bits);
     uint32_t old = atomicMax(&a.qwave[nb], wave + 2u);
     if (old < wave + 2u) list_append(a, nxt, nb);
+}
+
+// The same with the next wave's record the other parity of p (two records).
+__device__ __forceinline__ void activate(const SolveArgs& a, uint32_t nb, uint32_t wave,
+                                         uint32_t nxt, uint32_t p, uint32_t mask_bits,
+                                         float key) {
+    activate_np(a, nb, wave, nxt, p ^ 1u, mask_bits, key);
 }
 
 // Tiles the wave boundary did not select and left to this launch: each moves its activation
@@ -885,7 +891,7 @@ __device__ __forceinline__ void sweep_round(char* const shb, const int L) {
 
 template <int TW, int TH, int KIND, bool HOST>
 __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32_t nxt,
-                           uint32_t p, uint32_t job_slot, float* sh,
+                           uint32_t np, uint32_t job_slot, float* sh,
                            uint32_t* s_u32, float* s_f32, GhcmInst in_single) {
     constexpr int PW = TW + 2;
     constexpr int PH = TH + 2;
@@ -1047,9 +1053,9 @@ __device__ void sweep_tile(const SolveArgs& a, uint32_t g, uint32_t wave, uint32
         if (tid < 4 && s_u32[4 + tid]) {
             const int sd = tid;
             const uint32_t nb = tile_neighbour(in.tile_base, in.tiles_x, in.tiles_y, tx, ty, sd);
-            activate(a, nb, wave, nxt, p, border_bit_across(sd), s_f32[sd]);
+            activate_np(a, nb, wave, nxt, np, border_bit_across(sd), s_f32[sd]);
         }
-        if (tid == 64 && unfinished) activate(a, g, wave, nxt, p, kMaskOpenAll, tm);
+        if (tid == 64 && unfinished) activate_np(a, g, wave, nxt, np, kMaskOpenAll, tm);
     }
     if (tid == 128) {
         const uint32_t imp = s_u32[8] + s_u32[9] + s_u32[10] + s_u32[11];
@@ -1126,72 +1132,10 @@ void k_solve_sweep(SolveArgs a) {
                 g = a.jobs[j];
             }
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p ^ 1u, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
         if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// Boundary-free waves (launch_mode 4): every tile queued for a wave is solved in that wave,
-// so no selection runs and no wave-boundary kernel sits between two tile launches.  Wave w
-// (the launch argument) reads list[w % 3], whose shard counts are final because wave w-1
-// ended with its launch; every wave64 scans the counts in its lanes and finds a job's shard
-// with one ballot.  Activations go to list[(w+1) % 3].  The first work-group empties
-// list[(w+2) % 3] (consumed by wave w-1, filled by wave w+1), publishes progress for the
-// host, which keeps a few waves enqueued ahead and stops at the first empty one, and keeps
-// the wave counters.  The grid (n work-groups) loops over the list, so a wave of any length
-// is solved whole.
-// ---------------------------------------------------------------------------------------
-template <int TW, int TH, int KIND, int WPE = 1>
-__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
-void k_solve_sweep_bf(SolveArgs a, uint32_t wave) {
-    __shared__ float sh[sweep_lds_words<TW, TH>()];
-    __shared__ uint32_t s_u32[16];
-    __shared__ float s_f32[8];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const GhcmInst in0 = single_inst(a);
-    GhcmCtrl* ctrl = a.ctrl;
-    const uint32_t cur = wave % 3u;
-    const uint32_t nxt = cur == 2u ? 0u : cur + 1u;
-    const uint32_t emp = nxt == 2u ? 0u : nxt + 1u;
-    static_assert(kShards == 32, "one lane per shard, half a wave");
-    // Shard counts of this wave's list: lane l < 32 holds shard l (every lane reads a valid
-    // word, so the load needs no branch).
-    const uint32_t cnt = ctrl->count[cur][(uint32_t)(lane & (int)(kShards - 1)) * kCounterStride];
-    uint32_t inc = lane < (int)kShards ? cnt : 0u;
-#pragma unroll
-    for (int off = 1; off < (int)kShards; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off);
-        if (lane >= off) inc += y;
-    }
-    const uint32_t c = __shfl(inc, (int)kShards - 1);
-    // Exclusive prefix of the shard counts in lanes 0-31; lanes 32-63 never match a job.
-    const uint32_t pv = lane < (int)kShards ? inc - cnt : 0xFFFFFFFFu;
-    if (blockIdx.x == 0) {
-        if (tid < (int)kShards) ctrl->count[emp][tid * kCounterStride] = 0u;
-        if (tid == 0) {
-            ctrl->wave = wave;
-            if (c) {
-                __hip_atomic_fetch_add(&ctrl->busy_waves, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                ctrl->done = 1u;
-            }
-            if (a.rec) sys_store64(&a.rec->progress, ((unsigned long long)(c == 0u) << 32) | (wave + 1u));
-        }
-    }
-    if (c == 0u) return;
-    const uint32_t p = wave & 1u;
-    for (uint32_t j = blockIdx.x; j < c; j += gridDim.x) {
-        // Entry j of list[cur]: its shard is the last one whose prefix is <= j.
-        const uint32_t shd = (uint32_t)__popcll(__ballot(pv <= j)) - 1u;
-        const uint32_t base = __builtin_amdgcn_readlane(pv, (int)shd);
-        const uint32_t g = a.list[((size_t)cur * kShards + shd) * a.shard_cap + (j - base)];
-        if (tid == 0) claim_tile(a, g, p);
-        sweep_tile<TW, TH, KIND, false>(a, g, wave, nxt, p, j, sh, s_u32, s_f32, in0);
-        __syncthreads();
     }
 }
 
@@ -1227,6 +1171,303 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, 
     __syncthreads();
     return base + x - v;
 }
+
+// Block-wide exclusive scan over the four wave64s of a 256-thread work-group: each wave scans
+// with DPP (row shifts by 1, 2, 4 and 8 inside each row of 16 lanes, then row_bcast:15 and
+// row_bcast:31 across the rows; lanes without a source read 0, the identity), and the wave
+// totals go through part[0..3].  One barrier: a caller alternates between two part buffers,
+// so a scan never writes the words another thread may still be reading from the one before.
+__device__ __forceinline__ uint32_t block_excl_scan4(uint32_t v, uint32_t* part) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t x = v;
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    if (lane == 63) part[wv] = x;
+    __syncthreads();
+    uint32_t base = 0;
+    for (int w = 0; w < wv; ++w) base += part[w];
+    return base + x - v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Boundary-free waves (launch_mode 4): every tile queued for a wave is solved in that wave,
+// so no selection runs and no wave-boundary kernel sits between two tile launches.  Wave w
+// (the launch argument) reads list[w % 3], whose shard counts are final because wave w-1
+// ended with its launch; every wave64 scans the counts in its lanes and finds a job's shard
+// with one ballot.  Activations go to list[(w+1) % 3].  The first work-group empties
+// list[(w+2) % 3] (consumed by wave w-1, filled by wave w+1), publishes progress for the
+// host, which keeps a few waves enqueued ahead and stops at the first empty one, and keeps
+// the wave counters.  The grid (n work-groups) loops over the list, so a wave of any length
+// is solved whole.
+//
+// With SEL (launch_mode 5) the launch also makes the top-n cut itself: when list[w % 3] holds
+// more than n tiles, every work-group computes the same selection from the same records --
+// the rule of wave_select (two radix levels over key bits [30:19] and [18:7]; keys below the
+// threshold bin go, keys inside it go in list order up to the quota) -- so each wave solves
+// the tiles k_wave_end would have selected, without a wave-boundary kernel.  Activation
+// records then come in three like the lists (record index = list index): wave w reads record
+// w % 3, its activations write record (w+1) % 3, and it clears record (w+2) % 3 (read by wave
+// w-1, next written by wave w+1), so no record a wave reads changes while it runs.  The
+// launch's last kSelCarryGroups work-groups solve nothing: they carry the unselected tiles to
+// the next wave (key to record (w+1) % 3, entry to list (w+1) % 3), off the solving groups'
+// critical path.  Lists longer than kSelMax, or with more than kSelJobs selected tiles per
+// solving group, are solved whole as in launch_mode 4.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t kSelPerThread = 16;
+constexpr uint32_t kSelMax = kSelPerThread * kSweepThreads;     // 4096 entries
+constexpr uint32_t kSelJobs = 8;                                // selected tiles per solving group
+constexpr uint32_t kSelCarryGroups = 32;
+// LDS words of the selection scratch (histogram, scan partials, shard prefix, broadcast), laid
+// over the tile's box before the tile is staged.
+constexpr int kSelLdsWords = 4096 + 8 + (kShards + 1) + 4;
+
+template <int TW, int TH, int KIND, int WPE = 1, bool SEL = false>
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_solve_sweep_bf(SolveArgs a, uint32_t wave) {
+    __shared__ float sh[sweep_lds_words<TW, TH>()];
+    __shared__ uint32_t s_u32[16];
+    __shared__ float s_f32[8];
+    __shared__ uint32_t s_jobs[SEL ? kSelJobs : 1];
+    static_assert(!SEL || sweep_lds_words<TW, TH>() >= kSelLdsWords, "selection scratch fits the tile box");
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const GhcmInst in0 = single_inst(a);
+    GhcmCtrl* ctrl = a.ctrl;
+    const uint32_t cur = wave % 3u;
+    const uint32_t nxt = cur == 2u ? 0u : cur + 1u;
+    const uint32_t emp = nxt == 2u ? 0u : nxt + 1u;
+    static_assert(kShards == 32, "one lane per shard, half a wave");
+    // Shard counts of this wave's list: lane l < 32 holds shard l (every lane reads a valid
+    // word, so the load needs no branch).
+    const uint32_t cnt = ctrl->count[cur][(uint32_t)(lane & (int)(kShards - 1)) * kCounterStride];
+    const uint32_t n = SEL ? ctrl->n_select : 0u;
+    uint32_t inc = lane < (int)kShards ? cnt : 0u;
+#pragma unroll
+    for (int off = 1; off < (int)kShards; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+    }
+    const uint32_t c = __shfl(inc, (int)kShards - 1);
+    // Exclusive prefix of the shard counts in lanes 0-31; lanes 32-63 never match a job.
+    const uint32_t pv = lane < (int)kShards ? inc - cnt : 0xFFFFFFFFu;
+    if (blockIdx.x == 0) {
+        if (tid < (int)kShards) ctrl->count[emp][tid * kCounterStride] = 0u;
+        if (tid == 0) {
+            ctrl->wave = wave;
+            if (c) {
+                __hip_atomic_fetch_add(&ctrl->busy_waves, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                ctrl->done = 1u;
+            }
+            if (a.rec) sys_store64(&a.rec->progress, ((unsigned long long)(c == 0u) << 32) | (wave + 1u));
+        }
+    }
+    const uint32_t nsolve = SEL ? gridDim.x - kSelCarryGroups : gridDim.x;
+    if constexpr (SEL) {
+        // Clear record (w+2) % 3, every work-group a share.
+        const size_t T = a.tiles;
+        for (size_t t = (size_t)blockIdx.x * kSweepThreads + tid; t < T; t += (size_t)gridDim.x * kSweepThreads) {
+            a.key[(size_t)emp * T + t] = kInfBits;
+            a.mask[(size_t)emp * T + t] = 0u;
+        }
+    }
+    if (c == 0u) return;
+    const uint32_t p = wave & 1u;
+    // Record the activations of this wave write: the list index with SEL, else the parity.
+    const uint32_t np = SEL ? nxt : (p ^ 1u);
+    const bool cut = SEL && c > n && c <= kSelMax && n <= kSelJobs * nsolve;
+    if (!cut) {
+        if (SEL && blockIdx.x >= nsolve) return;
+        for (uint32_t j = blockIdx.x; j < c; j += nsolve) {
+            // Entry j of list[cur]: its shard is the last one whose prefix is <= j.
+            const uint32_t shd = (uint32_t)__popcll(__ballot(pv <= j)) - 1u;
+            const uint32_t base = __builtin_amdgcn_readlane(pv, (int)shd);
+            const uint32_t g = a.list[((size_t)cur * kShards + shd) * a.shard_cap + (j - base)];
+            if (!SEL && tid == 0) claim_tile(a, g, p);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, nxt, np, j, sh, s_u32, s_f32, in0);
+            __syncthreads();
+        }
+        return;
+    }
+    if constexpr (SEL) {
+        // ---- the top-n cut of wave_select, made by every work-group alike ----
+        // LDS scratch laid over the tile box: histogram, the keys of every tile (when the
+        // grid has at most 4096), scan partials, shard prefix, broadcast words.
+        uint32_t* const hist = reinterpret_cast<uint32_t*>(sh);
+        constexpr bool kAllKeys = sweep_lds_words<TW, TH>() >= kSelLdsWords + 4096;
+        uint32_t* const kall = hist + 4096;
+        uint32_t* const part = kall + (kAllKeys ? 4096 : 0);
+        uint32_t* const spre = part + 8;
+        uint32_t* const smisc = spre + kShards + 1;
+        const bool keys_lds = kAllKeys && a.tiles <= 4096u;
+        // Lane l < kShards of every wave holds shard l's exclusive prefix.
+        if (tid < (int)kShards) spre[tid] = pv;
+        if (tid == (int)kShards) spre[kShards] = c;
+        __syncthreads();
+        // Thread t holds entries [16 t, 16 t + 16) of the list (list order runs over threads).
+        const uint32_t i0 = (uint32_t)tid * kSelPerThread;
+        const uint32_t* key = a.key + (size_t)cur * a.tiles;
+        uint32_t eg[kSelPerThread], ek[kSelPerThread];
+        {
+            // Every entry's word first (a binary search over the shard prefix in LDS: the last
+            // shard with prefix <= i, which is never an empty one), then all list loads (and,
+            // for a small grid, the keys of all tiles beside them) unconditionally, indices past
+            // the list clamped to its last entry, so no load waits behind a branch or a load.
+            uint32_t wi[kSelPerThread];
+#pragma unroll
+            for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                const uint32_t i = min(i0 + u, c - 1u);
+                uint32_t sd = 0;
+#pragma unroll
+                for (uint32_t step = kShards / 2; step >= 1; step >>= 1)
+                    sd = spre[sd + step] <= i ? sd + step : sd;
+                wi[u] = (cur * kShards + sd) * a.shard_cap + (i - spre[sd]);
+            }
+            uint32_t kv[kSelPerThread];
+#pragma unroll
+            for (uint32_t u = 0; u < kSelPerThread; ++u) eg[u] = a.list[wi[u]];
+            // The first level's histogram is cleared while the loads are in flight.
+            for (int i = tid; i < 4096; i += kSweepThreads) hist[i] = 0u;
+            if (keys_lds) {
+#pragma unroll
+                for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                    const uint32_t t = (uint32_t)tid + kSweepThreads * u;
+                    kv[u] = key[min(t, a.tiles - 1u)];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                    const uint32_t t = (uint32_t)tid + kSweepThreads * u;
+                    if (t < a.tiles) kall[t] = kv[u];
+                }
+                __syncthreads();
+#pragma unroll
+                for (uint32_t u = 0; u < kSelPerThread; ++u) ek[u] = kall[eg[u]];
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kSelPerThread; ++u) ek[u] = key[eg[u]];
+                __syncthreads();
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kSelPerThread; ++u) ek[u] = i0 + u < c ? ek[u] : 0xFFFFFFFFu;
+        }
+        uint32_t lo_base = 0u, below = 0u;
+        for (int level = 0; level < 2; ++level) {
+            const int shift = level == 0 ? 19 : 7;
+            // A thread's keys are consecutive list entries, and near the front most share a
+            // bin: runs of one bin are counted locally and added with one atomic each.
+            uint32_t run_b = 0xFFFFFFFFu, run_n = 0u;
+#pragma unroll
+            for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                const uint32_t k = ek[u];
+                if (i0 + u < c && (level == 0 || (k >> 19) == (lo_base >> 19))) {
+                    const uint32_t b = (k >> shift) & 0xFFFu;
+                    if (b != run_b) {
+                        if (run_n) atomicAdd(&hist[run_b], run_n);
+                        run_b = b;
+                        run_n = 0u;
+                    }
+                    ++run_n;
+                }
+            }
+            if (run_n) atomicAdd(&hist[run_b], run_n);
+            __syncthreads();
+            constexpr int BPT = 4096 / kSweepThreads;
+            uint32_t hb[BPT], mine = 0;
+#pragma unroll
+            for (int j = 0; j < BPT; ++j) {
+                hb[j] = hist[BPT * tid + j];
+                mine += hb[j];
+            }
+            // (The scan's barrier also orders every read of this level's bins before the next
+            // level's clear below.)
+            const uint32_t excl = block_excl_scan4(mine, part + 4 * level) + below;
+            if (excl < n && excl + mine >= n) {           // the crossing bin is one of mine
+                uint32_t cum = excl, b = BPT * tid;
+                bool found = false;
+#pragma unroll
+                for (int j = 0; j < BPT; ++j) {
+                    if (!found && cum + hb[j] >= n) { b = BPT * tid + j; found = true; }
+                    if (!found) cum += hb[j];
+                }
+                smisc[2 * level] = b;
+                smisc[2 * level + 1] = cum;
+            }
+            if (level == 0)
+                for (int i = tid; i < 4096; i += kSweepThreads) hist[i] = 0u;
+            __syncthreads();
+            lo_base |= smisc[2 * level] << shift;
+            below = smisc[2 * level + 1];
+        }
+        // Keys below lo_base all go (`below` of them); keys in [lo_base, lo_base + 128) go in
+        // list order while the quota n - below lasts.  One scan of (below-count << 16 |
+        // in-bin count) gives both ranks: the selected entries before this thread's are its
+        // below-count prefix plus min(in-bin prefix, quota).
+        const uint32_t quota = n - below;
+        const uint32_t thr_hi = lo_base + (1u << 7);
+        // Job slots start empty (ordered before their assignment by the scan's barriers).
+        if (tid < (int)kSelJobs) s_jobs[tid] = kNoTile;
+        uint32_t nlo = 0, nib = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kSelPerThread; ++u) {
+            if (i0 + u >= c) continue;
+            if (ek[u] < lo_base) ++nlo;
+            else if (ek[u] < thr_hi) ++nib;
+        }
+        const uint32_t pre = block_excl_scan4((nlo << 16) | nib, part);      // part[0..3]: free again
+        uint32_t ib = pre & 0xFFFFu;
+        const uint32_t sel0 = (pre >> 16) + min(ib, quota);
+        uint32_t selm = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < kSelPerThread; ++u) {
+            if (i0 + u >= c) continue;
+            const uint32_t k = ek[u];
+            const bool go = k < lo_base || (k < thr_hi && ib++ < quota);
+            if (go) selm |= 1u << u;
+        }
+        const uint32_t nsel = (uint32_t)__popc(selm);
+        if (blockIdx.x < nsolve) {
+            // Selected tile of rank blockIdx.x + m * nsolve (list order) is this group's job m.
+            const uint32_t njobs = blockIdx.x < n ? (n - 1u - blockIdx.x) / nsolve + 1u : 0u;
+            for (uint32_t m = 0; m < njobs; ++m) {
+                const uint32_t r = blockIdx.x + m * nsolve;
+                if (r >= sel0 && r < sel0 + nsel) {
+                    const uint32_t want = r - sel0;
+                    uint32_t got = 0u;
+#pragma unroll
+                    for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                        if ((selm >> u) & 1u) {
+                            if (got == want) s_jobs[m] = eg[u];
+                            ++got;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t m = 0; m < njobs; ++m) {
+                const uint32_t g = s_jobs[m];
+                if (g >= a.tiles) continue;            // never taken: every rank has one owner
+                sweep_tile<TW, TH, KIND, false>(a, g, wave, nxt, np, blockIdx.x + m * nsolve, sh, s_u32, s_f32, in0);
+                __syncthreads();
+            }
+        } else {
+            // Carry group q takes the unselected entries of carry rank = q mod kSelCarryGroups.
+            const uint32_t q = blockIdx.x - nsolve;
+#pragma unroll
+            for (uint32_t u = 0; u < kSelPerThread; ++u) {
+                const uint32_t i = i0 + u;
+                if (i >= c || ((selm >> u) & 1u)) continue;
+                const uint32_t r = i - sel0 - (uint32_t)__popc(selm & ((1u << u) - 1u));
+                if (r % kSelCarryGroups == q)
+                    activate_np(a, eg[u], wave, nxt, nxt, kMaskOpenAll, __uint_as_float(ek[u]));
+            }
+        }
+    }
+}
+
 
 // Wave boundary, run by one work-group of NT threads: empty the consumed list and swap
 // (unless `first`), compute the shard prefix of the new list and, when it holds more than
@@ -1626,7 +1867,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_solve_sweep_fused(SolveArgs a
         for (uint32_t j = blockIdx.x; j < nj; j += gridDim.x) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p ^ 1u, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
         if (ncarry) carry_handed_off(a, nj, ncarry, wave, cur ^ 1u, p);
@@ -1797,7 +2038,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_persistent_sweep(SolveArgs a)
         for (uint32_t j = blockIdx.x; j < nj; j += G) {
             const uint32_t g = job_tile(a, s_prefix, cur, from_list, j);
             if (tid == 0) claim_tile(a, g, p);
-            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p, j, sh, s_u32, s_f32, in0);
+            sweep_tile<TW, TH, KIND, false>(a, g, wave, cur ^ 1u, p ^ 1u, j, sh, s_u32, s_f32, in0);
             __syncthreads();
         }
         // Tiles a boundary handed to the tile loop (none from this kernel's own selections,
@@ -2877,40 +3118,48 @@ SolveKernel get_solve_kernel(int tw, int th, int kind, int mode, bool host, bool
     return {nullptr, nullptr};
 }
 
-template <int TW, int TH, int KIND, int WPE>
+// grid: the solving work-groups; with SEL the launch adds kSelCarryGroups carrying ones.
+template <int TW, int TH, int KIND, int WPE, bool SEL>
 static hipError_t launch_bf_t(const SolveArgs& a, uint32_t wave, uint32_t grid, hipStream_t s,
                               hipEvent_t ev0, hipEvent_t ev1) {
+    const uint32_t g = SEL ? grid + kSelCarryGroups : grid;
     if (ev0 && ev1)
-        hipExtLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE>), dim3(grid), dim3(kSweepThreads),
+        hipExtLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE, SEL>), dim3(g), dim3(kSweepThreads),
                               0, s, ev0, ev1, 0, a, wave);
     else
-        hipLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE>), dim3(grid), dim3(kSweepThreads), 0,
+        hipLaunchKernelGGL((k_solve_sweep_bf<TW, TH, KIND, WPE, SEL>), dim3(g), dim3(kSweepThreads), 0,
                            s, a, wave);
     return hipGetLastError();
 }
 
-template <int TW, int TH, int KIND, int WPE>
+template <int TW, int TH, int KIND, int WPE, bool SEL>
 static int occ_bf_t() {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep_bf<TW, TH, KIND, WPE>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_solve_sweep_bf<TW, TH, KIND, WPE, SEL>,
                                                      kSweepThreads, 0) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
-template <int TW, int TH, int WPE>
+template <int TW, int TH, int WPE, bool SEL = false>
 static BfKernel pick_bf(int kind) {
     switch (kind) {
-        case 0: return {&launch_bf_t<TW, TH, 0, WPE>, &occ_bf_t<TW, TH, 0, WPE>};
-        case 1: return {&launch_bf_t<TW, TH, 1, WPE>, &occ_bf_t<TW, TH, 1, WPE>};
-        case 2: return {&launch_bf_t<TW, TH, 2, WPE>, &occ_bf_t<TW, TH, 2, WPE>};
-        case 3: return {&launch_bf_t<TW, TH, 3, WPE>, &occ_bf_t<TW, TH, 3, WPE>};
+        case 0: return {&launch_bf_t<TW, TH, 0, WPE, SEL>, &occ_bf_t<TW, TH, 0, WPE, SEL>};
+        case 1: return {&launch_bf_t<TW, TH, 1, WPE, SEL>, &occ_bf_t<TW, TH, 1, WPE, SEL>};
+        case 2: return {&launch_bf_t<TW, TH, 2, WPE, SEL>, &occ_bf_t<TW, TH, 2, WPE, SEL>};
+        case 3: return {&launch_bf_t<TW, TH, 3, WPE, SEL>, &occ_bf_t<TW, TH, 3, WPE, SEL>};
     }
     return {nullptr, nullptr};
 }
 
-BfKernel get_bf_kernel(int tw, int th, int kind, int mode, bool wide) {
+BfKernel get_bf_kernel(int tw, int th, int kind, int mode, bool wide, bool select) {
     if (mode != 2) return {nullptr, nullptr};
+    if (select) {
+        // The selection scratch is laid over the tile box: 64 x 64 and 64 x 32 tiles.
+        if (tw == 64 && th == 64) return wide ? pick_bf<64, 64, 4, true>(kind) : pick_bf<64, 64, 1, true>(kind);
+        if (tw == 64 && th == 32) return pick_bf<64, 32, 1, true>(kind);
+        return {nullptr, nullptr};
+    }
     if (tw == 64 && th == 64) return wide ? pick_bf<64, 64, 4>(kind) : pick_bf<64, 64, 1>(kind);
     if (tw == 64 && th == 32) return pick_bf<64, 32, 1>(kind);
     if (tw == 64 && th == 16) return pick_bf<64, 16, 1>(kind);

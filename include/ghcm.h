@@ -106,7 +106,12 @@ typedef struct ghcm_config {
                                        4 = boundary-free waves (sweep kernel): one
                                        tile-solve launch per wave and no wave-boundary
                                        kernel; every queued tile is solved in its wave
-                                       (no top-n cut), n sets the launch's work-groups */
+                                       (no top-n cut), n sets the launch's work-groups,
+                                       5 = boundary-free waves with the top-n cut made by
+                                       the tile-solve launch itself (the wave boundary's
+                                       selection rule; 64x64 and 64x32 sweep tiles, other
+                                       shapes run as 0); the allocation-free entry point
+                                       takes 0 and 4 only                               */
 } ghcm_config;
 
 /* Solve statistics (EikonalResult durations, src/eikonal.h:31-36, plus counters). */

@@ -62,8 +62,8 @@ def test_rings256_every_config(rings256, kernel, tile, sched):
 @pytest.mark.parametrize("tile", SWEEP_TILES)
 @pytest.mark.parametrize("sched,launch_mode", [(g.SCHED_DEVICE_TOPN, 0), (g.SCHED_DEVICE_TOPN, 1),
                                                (g.SCHED_DEVICE_TOPN, 2), (g.SCHED_DEVICE_TOPN, 3),
-                                               (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_HOST_PQ, 0),
-                                               (g.SCHED_HOST_PQ, 2)])
+                                               (g.SCHED_DEVICE_TOPN, 4), (g.SCHED_DEVICE_TOPN, 5),
+                                               (g.SCHED_HOST_PQ, 0), (g.SCHED_HOST_PQ, 2)])
 @pytest.mark.parametrize("cap", [0, 1, 2])
 def test_rings256_sweep_kernel(rings256, tile, sched, launch_mode, cap):
     cost, seeds, ref32, ref64 = rings256
@@ -118,6 +118,34 @@ def test_top_n_sizes(rings256, n_active, sweep):
     with g.Context(0, n_active=n_active, **cfg) as ctx:
         t, st = ctx.solve(cost, seeds)
     check(t, ref32, seeds, cost, label=f"n={n_active} sweep={sweep}")
+
+
+@pytest.mark.parametrize("tile", [(64, 64), (64, 32)])
+@pytest.mark.parametrize("n_active", [1, 3, 16, 64])
+def test_boundary_free_top_n(tile, n_active):
+    """launch_mode 5: the tile launch makes the top-n cut itself (every work-group applies the
+    boundary kernel's selection rule to the same records, and spare work-groups carry the
+    unselected tiles).  On a 1024^2 smooth field whose lists outgrow n, the solve matches the
+    oracle, visits about as many tiles as the boundary kernel's selection (launch_mode 0) and
+    no more than the boundary-free waves without a cut (launch_mode 4), which solve every
+    queued tile."""
+    speed = g.gen_smooth_speed(1024, 1024, 31, 32.0, 0.1, 10.0)
+    cost = g.cost_from_speed(speed)
+    seeds = g.gen_points(1024, 1024, 32, 3)
+    ref = oracle.fmm_f32(cost, seeds)
+    print(f"case boundary-free top-n tile={tile} n={n_active}", flush=True)
+    st = {}
+    for mode in (0, 4, 5):
+        with g.Context(0, tile_w=tile[0], tile_h=tile[1], kernel=g.KERNEL_SWEEP, inner_iter_cap=1,
+                       n_active=n_active, launch_mode=mode) as ctx:
+            t, st[mode] = ctx.solve(cost, seeds)
+        assert st[mode]["converged"] == 1
+        check(t, ref, seeds, cost, label=f"launch_mode {mode}")
+    v0, v4, v5 = (st[m]["tile_visits"] for m in (0, 4, 5))
+    print(f"  visits: mode0 {v0} mode4 {v4} mode5 {v5}; waves {st[0]['waves']} {st[4]['waves']} {st[5]['waves']}",
+          flush=True)
+    assert v5 <= v4
+    assert abs(v5 - v0) <= 0.1 * v0 + 8
 
 
 def test_long_lists_carried_by_the_tile_launch():
