@@ -29,6 +29,9 @@ Beside the headline the line carries:
              FIM;
   reference_kernels  the 4096^2 solve with the systolic (8x8, 32x8) and FIM (16x16) tile
              kernels at the reference's default wave size, each with its roofline;
+  launch_modes  the same 4096^2 solve and configuration in the boundary-free wave loops:
+             launch_mode 4 (every queued tile each wave) and 5 (the top-n cut made by the
+             tile launch), time per solve, waves, visits and per-launch time;
   volume     the 256^3 three-dimensional solve with the octant-sweep kernel (64x8x8
              tiles), its roofline and parity vs the FP64 3-D FMM, the in-tile-pass kernel
              (8x8x8 tiles) timed beside it, and the oracle's FP32 3-D FMM on one core.
@@ -505,6 +508,44 @@ def bench_reference_kernels(g, ctx, d_cost, d_time, seeds, steps: int) -> dict:
     return out
 
 
+def bench_launch_modes(g, ctx, d_cost, d_time, seeds, ref_t, steps: int) -> dict:
+    """The headline 4096^2 solve, same configuration, in the boundary-free wave loops
+    (no wave-boundary kernel): launch_mode 4 solves every queued tile each wave, 5 makes the
+    top-n cut inside the tile launch.  Time per solve, waves, visits, the tile launch's
+    average duration (HIP events) and the largest difference from the headline result."""
+    import numpy as np
+
+    saved = ctx.config.launch_mode
+    out = {}
+    for mode in (4, 5):
+        ctx.set_config(launch_mode=mode)
+
+        def step():
+            return ctx.solve_device(d_cost.ptr, g.FIELD_COST_F32, W, H, seeds, d_time.ptr)
+
+        for _ in range(2):
+            step()
+        t0 = time.perf_counter()
+        sts = [step() for _ in range(steps)]
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        t = d_time.download()
+        ctx.set_config(profile_kernels=1)
+        psts = [step() for _ in range(3)]
+        ctx.set_config(profile_kernels=0)
+        launches = sum(x["solve_launches"] for x in psts)
+        kern_us = sum(x["solve_kernel_us"] for x in psts)
+        fin = np.isfinite(ref_t)
+        out[f"launch_mode_{mode}"] = {
+            "ms_per_solve": round(ms, 4), "mcells_per_s": round(W * H / (ms / 1e3) / 1e6, 1),
+            "waves": sts[0]["waves"], "tile_visits": sts[0]["tile_visits"],
+            "avg_launch_us": round(kern_us / max(launches, 1), 3), "launches_per_step": launches / 3,
+            "max_abs_diff_vs_headline": float(np.max(np.abs(t[fin] - ref_t[fin]))),
+            "finiteness_mismatch": int(np.count_nonzero(np.isfinite(t) != fin))}
+        log(f"bench: launch_mode {mode}: {ms:.3f} ms per solve")
+    ctx.set_config(launch_mode=saved)
+    return out
+
+
 def bench_volume(g, steps: int, cpu: bool) -> dict:
     """The 3-D (K = 3) solve on the 256^3 smooth volume: Mcells/s, the 3-D tile kernel's
     roofline, parity against the oracle's FP32 and FP64 3-D FMM, the oracle timed on one core."""
@@ -608,6 +649,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
     ap.add_argument("--no-volume", action="store_true", help="skip the 3-D volume line")
+    ap.add_argument("--no-launch-modes", action="store_true",
+                    help="skip the headline solve in the boundary-free wave loops (launch_mode 4 and 5)")
     ap.add_argument("--no-reference-kernels", action="store_true",
                     help="skip the 4096^2 solve with the systolic and FIM tile kernels")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -716,6 +759,8 @@ def main() -> int:
 
     if not args.no_reference_kernels:
         out["reference_kernels"] = bench_reference_kernels(g, ctx, d_cost, d_time, seeds, 3)
+    if not args.no_launch_modes and kernel == g.KERNEL_SWEEP and args.launch_mode == 0:
+        out["launch_modes"] = bench_launch_modes(g, ctx, d_cost, d_time, seeds, t_gpu, args.steps)
 
     plain_cfg = dict(tile_w=tw, tile_h=th, kernel=kernel, n_active=args.n_active,
                      inner_iter_cap=args.inner_cap, waves_per_poll=args.waves_per_poll,
