@@ -55,14 +55,18 @@ extern "C" int ghcm_tune_default_space(int cus, ghcm_tune_candidate* out, uint32
                 put(out, cap, &n, t[0], t[1], GHCM_KERNEL_FIM, na, 0u, mode);
             }
         }
-    // The sweep kernel's other wave loops: one persistent cooperative launch (1) and
-    // boundary-free waves (4, every queued tile each wave, n sets the grid).
-    for (uint32_t mode : {1u, 4u})
+    // The sweep kernel's other wave loops: one persistent cooperative launch (1),
+    // boundary-free waves (4, every queued tile each wave, n sets the grid) and boundary-free
+    // waves with the top-n cut made by the tile launch (5; 64x64 and 64x32 tiles only, the
+    // others would run as 0).
+    for (uint32_t mode : {1u, 4u, 5u})
         for (uint32_t m : mult) {
             const uint32_t na = m * (uint32_t)cus;
-            for (const auto& t : large)
+            for (const auto& t : large) {
+                if (mode == 5u && t[0] != 64u) continue;
                 for (uint32_t inner : {1u, 0u})
                     put(out, cap, &n, t[0], t[1], GHCM_KERNEL_SWEEP, na, inner, mode);
+            }
         }
     *n_out = n < cap ? n : cap;
     if (n > cap) return fail(GHCM_ERR_INVALID_ARG, "tune space has %u candidates, capacity %u", n, cap);
