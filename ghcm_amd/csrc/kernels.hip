@@ -1279,7 +1279,7 @@ void k_solve_sweep_bf(SolveArgs a, uint32_t wave) {
     const uint32_t p = wave & 1u;
     // Record the activations of this wave write: the list index with SEL, else the parity.
     const uint32_t np = SEL ? nxt : (p ^ 1u);
-    const bool cut = SEL && c > n && c <= kSelMax && n <= kSelJobs * nsolve;
+    const bool cut = SEL && n > 0u && c > n && c <= kSelMax && n <= kSelJobs * nsolve;
     if (!cut) {
         if (SEL && blockIdx.x >= nsolve) return;
         for (uint32_t j = blockIdx.x; j < c; j += nsolve) {
