@@ -1139,53 +1139,58 @@ void k_solve_sweep(SolveArgs a) {
     }
 }
 
-// Block-wide exclusive scan of v over NT threads (NT a multiple of 64, at most 1024;
-// part[] holds at least 64 + NT/64 words); returns the exclusive prefix of this thread
-// and stores the block total in *total.  Wave scans by __shfl_up, then one wave scans the
-// wave totals: two barriers instead of two per doubling step.
-template <int NT>
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, uint32_t* total) {
-    constexpr int NWV = NT / 64;
-    static_assert(NT % 64 == 0 && NWV <= 64, "whole waves");
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) part[wv] = x;
-    __syncthreads();
-    if (tid < 64) {
-        uint32_t t = tid < NWV ? part[tid] : 0u;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = __shfl_up(t, off);
-            if (tid >= off) t += y;
-        }
-        if (tid < NWV) part[64 + tid] = t;
-    }
-    __syncthreads();
-    const uint32_t base = wv ? part[64 + wv - 1] : 0u;
-    *total = part[64 + NWV - 1];
-    __syncthreads();
-    return base + x - v;
-}
-
-// Block-wide exclusive scan over the four wave64s of a 256-thread work-group: each wave scans
-// with DPP (row shifts by 1, 2, 4 and 8 inside each row of 16 lanes, then row_bcast:15 and
-// row_bcast:31 across the rows; lanes without a source read 0, the identity), and the wave
-// totals go through part[0..3].  One barrier: a caller alternates between two part buffers,
-// so a scan never writes the words another thread may still be reading from the one before.
-__device__ __forceinline__ uint32_t block_excl_scan4(uint32_t v, uint32_t* part) {
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    uint32_t x = v;
+// Inclusive sums by DPP, without a cross-lane memory round trip: row shifts by 1, 2, 4 and 8
+// inside each row of 16 lanes (lanes without a source read 0, the identity), then lane 15 of
+// rows 0 and 2 added to rows 1 and 3 (row_bcast:15): each 32-lane half of the wave scanned on
+// its own.  The whole wave adds lane 31 to rows 2 and 3 as well (row_bcast:31).  Every lane
+// of the wave must be active.
+__device__ __forceinline__ uint32_t half_incl_scan_dpp(uint32_t x) {
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);   // row_shr:1
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);   // row_shr:2
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);   // row_shr:4
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);   // row_shr:8
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t x) {
+    x = half_incl_scan_dpp(x);
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
+// Block-wide exclusive scan of v over NT threads (NT a multiple of 64, at most 1024;
+// part[] holds at least 64 + NT/64 words); returns the exclusive prefix of this thread
+// and stores the block total in *total.  Each wave scans by DPP, lane 63 posts the wave's
+// total and every thread adds up the totals of the waves before its own: two barriers (the
+// second leaves part[] free for the next scan).
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* part, uint32_t* total) {
+    constexpr int NWV = NT / 64;
+    static_assert(NT % 64 == 0 && NWV <= 64, "whole waves");
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t x = wave_incl_scan_dpp(v);
+    if (lane == 63) part[wv] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWV; ++w) {
+        const uint32_t t = part[w];
+        tot += t;
+        base += w < wv ? t : 0u;
+    }
+    *total = tot;
+    __syncthreads();
+    return base + x - v;
+}
+
+// Block-wide exclusive scan over the four wave64s of a 256-thread work-group: each wave scans
+// by DPP (wave_incl_scan_dpp) and the wave totals go through part[0..3].  One barrier: a caller
+// alternates between two part buffers, so a scan never writes the words another thread may
+// still be reading from the one before.
+__device__ __forceinline__ uint32_t block_excl_scan4(uint32_t v, uint32_t* part) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t x = wave_incl_scan_dpp(v);
     if (lane == 63) part[wv] = x;
     __syncthreads();
     uint32_t base = 0;
@@ -1245,13 +1250,9 @@ void k_solve_sweep_bf(SolveArgs a, uint32_t wave) {
     // word, so the load needs no branch).
     const uint32_t cnt = ctrl->count[cur][(uint32_t)(lane & (int)(kShards - 1)) * kCounterStride];
     const uint32_t n = SEL ? ctrl->n_select : 0u;
-    uint32_t inc = lane < (int)kShards ? cnt : 0u;
-#pragma unroll
-    for (int off = 1; off < (int)kShards; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off);
-        if (lane >= off) inc += y;
-    }
-    const uint32_t c = __shfl(inc, (int)kShards - 1);
+    // Lanes 0-31 scan the counts (lanes 32-63 scan zeros).
+    const uint32_t inc = half_incl_scan_dpp(lane < (int)kShards ? cnt : 0u);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)inc, (int)kShards - 1);
     // Exclusive prefix of the shard counts in lanes 0-31; lanes 32-63 never match a job.
     const uint32_t pv = lane < (int)kShards ? inc - cnt : 0xFFFFFFFFu;
     if (blockIdx.x == 0) {
@@ -1502,14 +1503,9 @@ __device__ int wave_select(const SolveArgs& a, bool first, uint32_t* hist, uint3
     // the other list becomes current.
     const uint32_t cur = first ? cur0 : (cur0 ^ 1u);
     if (tid < 64) {
-        // Lanes 0-31 scan list 0's counts, lanes 32-63 list 1's.
+        // Lanes 0-31 scan list 0's counts, lanes 32-63 list 1's (wave 0, every lane active).
         const uint32_t sl = (uint32_t)tid & 31u, lst = (uint32_t)tid >> 5;
-        uint32_t x = cnt;
-#pragma unroll
-        for (uint32_t off = 1; off < kShards; off <<= 1) {
-            const uint32_t y = __shfl_up(x, off);
-            if (sl >= off) x += y;
-        }
+        const uint32_t x = half_incl_scan_dpp(cnt);
         if (lst == cur) {
             s_prefix[sl + 1] = x;
             if (sl == 0) s_prefix[0] = 0u;
