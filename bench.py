@@ -642,7 +642,8 @@ def main() -> int:
     ap.add_argument("--inner-cap", type=int, default=1, help="in-tile rounds per visit (0 = converge)")
     ap.add_argument("--waves-per-poll", type=int, default=4)
     ap.add_argument("--launch-mode", type=int, default=0,
-                    help="0 per-wave launches, 1 persistent, 2 graph replay, 3 fused boundary, 4 boundary-free")
+                    help="0 per-wave launches, 1 persistent, 2 graph replay, 3 fused boundary, 4 boundary-free, "
+                         "5 boundary-free with the top-n cut made by the tile launch")
     ap.add_argument("--cpu-trials", type=int, default=5)
     ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision report")
     ap.add_argument("--no-cpu-baseline", action="store_true")
