@@ -638,8 +638,9 @@ int solve_core_body(ghcm_ctx* ctx, int kind, const HostInst* hi, uint32_t n_inst
     // The instance table and the seeds are read by the kernels from mapped pinned memory:
     // the whole solve issues no runtime copy.
     HIP_TRY(hipEventRecord(ctx->ev_begin, st));
-    HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st));
-    HIP_TRY(launch_edges_init(a, 0, T, tw, th, td, kind, st));
+    // The time fields at +INF and, for the sweep kernels, the tiles' edge columns in one launch.
+    HIP_TRY(launch_solve_init(a, ctx->h_inst.dev_as<GhcmInst>(), n_inst, max_cells, n_sel, d_totals, st,
+                              tw, th, td, kind));
     if (three_d)
         HIP_TRY(launch_seed3d(a, ctx->h_seeds.dev_as<uint32_t>(), (uint32_t)n_seeds_total, tw, th, td, st));
     else

@@ -69,10 +69,13 @@ hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds_xyi, uint
 hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
                               uint32_t gen, hipStream_t s);
 hipError_t launch_wake(GhcmCtrl* ctrl, hipStream_t s);
-// h_inst: device view of the mapped host instance table.
+// h_inst: device view of the mapped host instance table.  tw != 0: also the edge columns of
+// every instance's tiles (tw x th x td tiles, cost kind `kind`; nothing when a.edge is null),
+// as launch_edges_init would.
 hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_t n_inst,
                              uint64_t max_cells, uint32_t n_select, unsigned long long* totals,
-                             hipStream_t s);
+                             hipStream_t s, uint32_t tw = 0, uint32_t th = 0, uint32_t td = 1,
+                             int kind = 0);
 hipError_t launch_publish_totals(const unsigned long long* totals, const GhcmCtrl* ctrl,
                                  HostRec* rec, uint32_t tag, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);

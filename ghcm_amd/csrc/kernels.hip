@@ -1755,8 +1755,37 @@ __global__ __launch_bounds__(1024) void k_wave_end(SolveArgs a, uint32_t first) 
 // Per-solve initialisation in one launch (grid.y over instances): the control block and
 // totals zeroed, the instance table copied from mapped host memory, every tile's queue
 // state reset and mapped to its instance, every time field set to +INF.
+// Edge columns of one instance's tiles (the sweep kernels' halo source; eh = th * td rows per
+// column, row r at (y, z) = (r mod th, r / th) of the tile): time +INF and the cost of the same
+// cells (+INF past the grid).  Entry e of the instance is row e mod eh of tile e / (2 eh)'s left
+// column, or of its right one when e mod 2eh >= eh.  Threads [first, ...) with `stride`.
+template <int KIND>
+__device__ void init_edges(const SolveArgs& a, const GhcmInst& in, uint32_t nt, uint32_t tw,
+                           uint32_t th, uint32_t td, uint32_t first, uint32_t stride) {
+    const uint32_t eh = th * td, e2 = 2u * eh;
+    const uint32_t ne = nt * e2;
+    for (uint32_t e = first; e < ne; e += stride) {
+        const uint32_t t = e / e2, j = e - t * e2;
+        const uint32_t right = j >= eh ? 1u : 0u, r = j - right * eh;
+        const uint32_t tx = t % in.tiles_x, ty = (t / in.tiles_x) % in.tiles_y;
+        const uint32_t tz = t / (in.tiles_x * in.tiles_y);
+        const uint32_t x = tx * tw + (right ? tw - 1 : 0u), y = ty * th + r % th, z = tz * td + r / th;
+        float c = __builtin_inff();
+        if (x < in.w && y < in.h && z < in.d)
+            c = load_cost_at<KIND>(static_cast<const char*>(in.field) +
+                                   (((size_t)z * in.h + y) * in.w + x) * field_bytes<KIND>());
+        const size_t o = (size_t)(in.tile_base + t) * e2 + j;
+        gmem(a.edge)[o] = __builtin_inff();
+        gmem(const_cast<float*>(a.cedge))[o] = c;
+    }
+}
+
+// tw != 0 (with a.edge set): the edge columns of every tile are initialised too, each
+// instance's by the work-groups that fill its time field (no instance lookup per tile).
+template <int KIND>
 __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_inst,
-                             uint32_t n_select, unsigned long long* totals) {
+                             uint32_t n_select, unsigned long long* totals, uint32_t tw,
+                             uint32_t th, uint32_t td) {
     if (blockIdx.x == 0 && blockIdx.y == 0) {
         uint32_t* c = reinterpret_cast<uint32_t*>(a.ctrl);
         for (uint32_t i = threadIdx.x; i < sizeof(GhcmCtrl) / 4; i += blockDim.x) c[i] = 0u;
@@ -1789,6 +1818,8 @@ __global__ void k_solve_init(SolveArgs a, const GhcmInst* h_inst, uint32_t n_ins
         const size_t cells = (size_t)in.w * in.h * in.d;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += stride)
             tm[i] = kInfBits;
+        if (tw && a.edge)
+            init_edges<KIND>(a, in, nt, tw, th, td, blockIdx.x * blockDim.x + threadIdx.x, stride);
     }
 }
 
@@ -3230,13 +3261,19 @@ hipError_t launch_wake(GhcmCtrl* ctrl, hipStream_t s) {
 
 hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_t n_inst,
                              uint64_t max_cells, uint32_t n_select, unsigned long long* totals,
-                             hipStream_t s) {
+                             hipStream_t s, uint32_t tw, uint32_t th, uint32_t td, int kind) {
+    if (tw && a.edge && th * td != a.edge_h) return hipErrorInvalidValue;
     uint64_t bx = (max_cells + 256ull * 16 - 1) / (256ull * 16);
     if (bx < 1) bx = 1;
     if (bx > 1024) bx = 1024;
     const uint32_t by = n_inst < 65535u ? (n_inst ? n_inst : 1u) : 65535u;
-    hipLaunchKernelGGL(k_solve_init, dim3((unsigned)bx, by), dim3(256), 0, s, a, h_inst, n_inst,
-                       n_select, totals);
+    const dim3 grid((unsigned)bx, by), block(256);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_solve_init<0>, grid, block, 0, s, a, h_inst, n_inst, n_select, totals, tw, th, td); break;
+        case 1: hipLaunchKernelGGL(k_solve_init<1>, grid, block, 0, s, a, h_inst, n_inst, n_select, totals, tw, th, td); break;
+        case 2: hipLaunchKernelGGL(k_solve_init<2>, grid, block, 0, s, a, h_inst, n_inst, n_select, totals, tw, th, td); break;
+        default: hipLaunchKernelGGL(k_solve_init<3>, grid, block, 0, s, a, h_inst, n_inst, n_select, totals, tw, th, td); break;
+    }
     return hipGetLastError();
 }
 
