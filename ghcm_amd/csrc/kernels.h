@@ -57,21 +57,19 @@ constexpr int kNoAllocMaxDevices = 64;
 hipError_t noalloc_scratch_init(int device);
 void* noalloc_scratch(int device);
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
-// Edge columns of tiles [t0, t0 + count) after the time fields were set to +INF (sweep
-// kernel; nothing when a.edge is null).  Launched before the seeds.
-hipError_t launch_edges_init(const SolveArgs& a, uint32_t t0, uint32_t count, uint32_t tw,
-                             uint32_t th, uint32_t td, int kind, hipStream_t s);
 hipError_t launch_tile_inst(uint32_t* tinst, const GhcmInst* inst, uint32_t n_inst, hipStream_t s);
 hipError_t launch_reduce_stats(const uint32_t* tstat, uint32_t tiles, unsigned long long* out,
                               hipStream_t s);
 hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds_xyi, uint32_t n_seeds,
                              uint32_t tile_w, uint32_t tile_h, hipStream_t s);
+// One admitted request: its slot entry, tile states and tile map, its time field at +INF and,
+// with tw != 0 (th rows per edge column, cost kind `kind`), its tiles' edge columns.
 hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
-                              uint32_t gen, hipStream_t s);
+                              uint32_t gen, hipStream_t s, uint32_t tw = 0, uint32_t th = 0,
+                              int kind = 0);
 hipError_t launch_wake(GhcmCtrl* ctrl, hipStream_t s);
 // h_inst: device view of the mapped host instance table.  tw != 0: also the edge columns of
-// every instance's tiles (tw x th x td tiles, cost kind `kind`; nothing when a.edge is null),
-// as launch_edges_init would.
+// every instance's tiles (tw x th x td tiles, cost kind `kind`; nothing when a.edge is null).
 hipError_t launch_solve_init(const SolveArgs& a, const GhcmInst* h_inst, uint32_t n_inst,
                              uint64_t max_cells, uint32_t n_select, unsigned long long* totals,
                              hipStream_t s, uint32_t tw = 0, uint32_t th = 0, uint32_t td = 1,

@@ -1936,33 +1936,6 @@ __device__ __forceinline__ void seed_edges(const SolveArgs& a, const GhcmInst& i
     if (cx == tw - 1) gmem(a.edge)[(size_t)t * 2 * th + th + r] = 0.0f;
 }
 
-// Edge columns of tiles [t0, t0 + count): time +INF (the time fields were just set to +INF)
-// and the cost of the same cells (+INF past the grid).  One work-group of 2 * th threads per
-// tile, one cell each.
-template <int KIND>
-__global__ void k_edges_init(SolveArgs a, uint32_t t0, uint32_t count, uint32_t tw, uint32_t th, uint32_t td) {
-    // An edge column has th * td rows: row r is (y, z) = (r mod th, r / th) of the tile (td = 1
-    // in two dimensions).
-    const uint32_t eh = th * td;
-    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
-        const uint32_t g = t0 + k;
-        const uint32_t ii = a.n_inst == 1 ? 0u : a.tinst[g];
-        const GhcmInst in = a.inst[ii];
-        const uint32_t lt = g - in.tile_base;
-        const uint32_t tx = lt % in.tiles_x, ty = (lt / in.tiles_x) % in.tiles_y;
-        const uint32_t tz = lt / (in.tiles_x * in.tiles_y);
-        const uint32_t j = threadIdx.x, right = j >= eh ? 1u : 0u, r = j - right * eh;
-        const uint32_t x = tx * tw + (right ? tw - 1 : 0u), y = ty * th + r % th, z = tz * td + r / th;
-        float c = __builtin_inff();
-        if (x < in.w && y < in.h && z < in.d)
-            c = load_cost_at<KIND>(static_cast<const char*>(in.field) +
-                                   (((size_t)z * in.h + y) * in.w + x) * field_bytes<KIND>());
-        const size_t o = (size_t)g * 2 * eh + j;
-        gmem(a.edge)[o] = __builtin_inff();
-        gmem(const_cast<float*>(a.cedge))[o] = c;
-    }
-}
-
 // Serving admission: seeds of newly admitted instances, (x, y, slot) triples.  Queues the
 // tiles of each seed and its 4 neighbours for the wave after the current one (they join
 // list[cur^1] like any activation) and clears `done` so an idle loop resumes.
@@ -1997,7 +1970,9 @@ __global__ void k_wake(GhcmCtrl* ctrl) {
 
 // Serving admission: the slot's instance entry written, its tile range reset and mapped
 // to the slot.
-__global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t gen) {
+template <int KIND>
+__global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t gen, uint32_t tw,
+                              uint32_t th) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const_cast<GhcmInst*>(a.inst)[slot] = in;
         const_cast<uint32_t*>(a.sgen)[slot] = gen;
@@ -2016,6 +1991,12 @@ __global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t 
         a.tstat[2 * (size_t)a.tiles + t] = 0u;
         const_cast<uint32_t*>(a.tinst)[t] = slot;
     }
+    // The request's time field at +INF and, for the sweep kernel, its tiles' edge columns.
+    const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    auto* tm = gmem(reinterpret_cast<uint32_t*>(in.time));
+    const size_t cells = (size_t)in.w * in.h * in.d;
+    for (size_t i = first; i < cells; i += stride) tm[i] = kInfBits;
+    if (tw && a.edge) init_edges<KIND>(a, in, count, tw, th, 1u, first, stride);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -3199,21 +3180,6 @@ hipError_t launch_wave_end(const SolveArgs& a, uint32_t first, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_edges_init(const SolveArgs& a, uint32_t t0, uint32_t count, uint32_t tw,
-                             uint32_t th, uint32_t td, int kind, hipStream_t s) {
-    if (!a.edge || count == 0) return hipSuccess;
-    if (2 * th * td > 1024 || th * td != a.edge_h) return hipErrorInvalidValue;
-    const unsigned blocks = count < 65536u ? count : 65536u;
-    const dim3 grid(blocks), block(2 * th * td);
-    switch (kind) {
-        case 0: hipLaunchKernelGGL(k_edges_init<0>, grid, block, 0, s, a, t0, count, tw, th, td); break;
-        case 1: hipLaunchKernelGGL(k_edges_init<1>, grid, block, 0, s, a, t0, count, tw, th, td); break;
-        case 2: hipLaunchKernelGGL(k_edges_init<2>, grid, block, 0, s, a, t0, count, tw, th, td); break;
-        default: hipLaunchKernelGGL(k_edges_init<3>, grid, block, 0, s, a, t0, count, tw, th, td); break;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
     if (n == 0) return hipSuccess;
     size_t blocks = (n + 255) / 256;
@@ -3245,12 +3211,21 @@ hipError_t launch_seed_serve(const SolveArgs& a, const uint32_t* seeds, uint32_t
 }
 
 hipError_t launch_admit_tiles(const SolveArgs& a, const GhcmInst& in, uint32_t slot,
-                              uint32_t gen, hipStream_t s) {
+                              uint32_t gen, hipStream_t s, uint32_t tw, uint32_t th, int kind) {
+    if (tw && a.edge && th != a.edge_h) return hipErrorInvalidValue;
     const uint32_t count = inst_tiles(in);
-    unsigned blocks = (count + 255) / 256;
+    const uint64_t cells = (uint64_t)in.w * in.h * in.d;
+    const uint64_t bt = (count + 255) / 256, bc = (cells + 256ull * 16 - 1) / (256ull * 16);
+    uint64_t blocks = bt > bc ? bt : bc;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_admit_tiles, dim3(blocks), dim3(256), 0, s, a, in, slot, gen);
+    const dim3 grid((unsigned)blocks), block(256);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_admit_tiles<0>, grid, block, 0, s, a, in, slot, gen, tw, th); break;
+        case 1: hipLaunchKernelGGL(k_admit_tiles<1>, grid, block, 0, s, a, in, slot, gen, tw, th); break;
+        case 2: hipLaunchKernelGGL(k_admit_tiles<2>, grid, block, 0, s, a, in, slot, gen, tw, th); break;
+        default: hipLaunchKernelGGL(k_admit_tiles<3>, grid, block, 0, s, a, in, slot, gen, tw, th); break;
+    }
     return hipGetLastError();
 }
 

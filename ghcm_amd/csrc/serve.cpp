@@ -484,10 +484,9 @@ int ghcm_server::admit(std::vector<Request*>& batch, uint32_t enqueued) {
         g.tiles_z = 1;
         r->gen = next_gen++;
         if (next_gen == 0) next_gen = 1;
-        HIP_TRY(launch_admit_tiles(a, g, r->slot, r->gen, stream));
-        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(g.time), cells, kInfBits, stream));
-        // edge columns of the slot's tiles (sweep kernel), after its entry and tile map
-        HIP_TRY(launch_edges_init(a, g.tile_base, g.tiles_x * g.tiles_y, tw, th, 1u, kind, stream));
+        // The slot's entry, tile states and tile map, its time field at +INF and its tiles'
+        // edge columns (sweep kernel) in one launch.
+        HIP_TRY(launch_admit_tiles(a, g, r->slot, r->gen, stream, tw, th, kind));
         for (size_t s = 0; s + 1 < r->seeds.size(); s += 2) {
             hs[k++] = r->seeds[s];
             hs[k++] = r->seeds[s + 1];
