@@ -168,8 +168,10 @@ def accuracy_vs_fp64(results, refs64) -> dict:
             "results": len(rel)}
 
 
-def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
-    """512 uneven instances in one device wave loop vs the same solves one at a time."""
+def bench_batch(g, plain_cfg: dict, steps: int, autotune: bool = True) -> dict:
+    """512 uneven instances in one device wave loop vs the same solves one at a time.
+    autotune=False skips the search and solves with BATCH_CANDIDATES[1], the configuration
+    the search picks on MI355X (for counter runs that should see only the batch's kernels)."""
     import numpy as np
     from ghcm_amd import workloads
     t0 = time.perf_counter()
@@ -186,8 +188,13 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             ctx.solve_device(a[0], g.FIELD_COST_F32, a[2], a[3], a[4], a[1])
         t_plain = time.perf_counter() - t0
         ref = [dt.download() for _, dt in dev]          # the one-at-a-time results
-        best, res = ctx.autotune(g.FIELD_COST_F32, args, BATCH_CANDIDATES, reps=1)
-        log(f"bench: batch autotune -> {best}")
+        if autotune:
+            best, res = ctx.autotune(g.FIELD_COST_F32, args, BATCH_CANDIDATES, reps=1)
+            log(f"bench: batch autotune -> {best}")
+        else:
+            best, res = dict(BATCH_CANDIDATES[1]), []
+            ctx.set_config(**best)
+            log(f"bench: batch configuration (no search) -> {best}")
         prepared = ctx.prepare_batch(args)          # the instance array, built once
         ctx.solve_batch(g.FIELD_COST_F32, prepared)
         ts = []
@@ -238,7 +245,7 @@ def bench_batch(g, plain_cfg: dict, steps: int) -> dict:
             "speedup_vs_plain": round(t_plain / t_batch, 2),
             "batch_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
             "batch_mcells_per_s": round(cells / t_batch / 1e6, 1),
-            "waves": st["waves"], "converged": st["converged"],
+            "waves": st["waves"], "tile_visits": st["tile_visits"], "converged": st["converged"],
             "autotuned": {k: best[k] for k in keys},
             "autotune_device_ms": {f"{c['tile_w']}x{c['tile_h']}/k{c['kernel']}/n{c['n_active']}/"
                                    f"cap{c['inner_iter_cap']}/lm{c['launch_mode']}":
@@ -648,6 +655,9 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision report")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve line")
+    ap.add_argument("--no-batch-autotune", action="store_true",
+                    help="solve the batch with the configuration the autotuner picks on MI355X "
+                         "(64x64 sweep, n = 65,536) without running the search")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving line")
     ap.add_argument("--no-volume", action="store_true", help="skip the 3-D volume line")
     ap.add_argument("--no-launch-modes", action="store_true",
@@ -772,7 +782,7 @@ def main() -> int:
     contrast = None if args.no_precision else contrast_refs_async(g, bg)
     if not args.no_batch:
         log("bench: batched solves")
-        out["batched"] = bench_batch(g, plain_cfg, 3)
+        out["batched"] = bench_batch(g, plain_cfg, 3, autotune=not args.no_batch_autotune)
         log(f"bench: batched {out['batched']['solves_per_s']} solves/s, "
             f"{out['batched']['speedup_vs_plain']}x plain")
     if not args.no_serve:
