@@ -19,7 +19,8 @@ Beside the headline the line carries:
   batched    512 uneven instances in one device wave loop (autotuned over tile shapes,
              n, round caps and launch modes) vs the same solves launched one at a time,
              the L_inf of every batched and one-at-a-time result vs the oracle's FP64 FMM,
-             and the oracle FMM on all cores;
+             the batch's algorithmic bytes over its wall time (fraction of HBM peak), and
+             the oracle FMM on all cores;
   serving    the continuous-batching server on 2,048 requests resident in HBM and 256 on
              host buffers: saturated solves/s vs one-at-a-time, p50/p99 latency under
              Poisson arrivals at 50 % and 80 % of the saturated rate, served results vs
@@ -246,6 +247,11 @@ def bench_batch(g, plain_cfg: dict, steps: int, autotune: bool = True) -> dict:
             "batch_ms": round(t_batch * 1e3, 3), "plain_ms": round(t_plain * 1e3, 3),
             "batch_mcells_per_s": round(cells / t_batch / 1e6, 1),
             "waves": st["waves"], "tile_visits": st["tile_visits"], "converged": st["converged"],
+            # Algorithmic bytes of the whole batched solve (visits x bytes per visit) over its
+            # wall time: the batch's average rate, boundary launches and gaps included.
+            "algorithmic_gb_per_s": round(st["tile_visits"] * st["bytes_per_visit"] / t_batch / 1e9, 1),
+            "frac_of_hbm_peak": round(st["tile_visits"] * st["bytes_per_visit"] / t_batch / 1e9
+                                      / HBM_PEAK_GBS, 4),
             "autotuned": {k: best[k] for k in keys},
             "autotune_device_ms": {f"{c['tile_w']}x{c['tile_h']}/k{c['kernel']}/n{c['n_active']}/"
                                    f"cap{c['inner_iter_cap']}/lm{c['launch_mode']}":
