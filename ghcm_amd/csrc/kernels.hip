@@ -1968,8 +1968,9 @@ __global__ void k_wake(GhcmCtrl* ctrl) {
     if (threadIdx.x == 0) ctrl->done = 0u;
 }
 
-// Serving admission: the slot's instance entry written, its tile range reset and mapped
-// to the slot.
+// Serving admission in one launch: the slot's instance entry written, its tile range reset
+// and mapped to the slot, its time field set to +INF and (tw != 0, sweep kernel) its tiles'
+// edge columns initialised from the instance record passed by value.
 template <int KIND>
 __global__ void k_admit_tiles(SolveArgs a, GhcmInst in, uint32_t slot, uint32_t gen, uint32_t tw,
                               uint32_t th) {
